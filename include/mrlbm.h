@@ -1,0 +1,167 @@
+/*
+ * mrlbm.h - C ABI of the MI355X moment-representation LBM step.
+ *
+ * One context = one lattice (an "instance") resident in HBM: the 10 stored
+ * moments per node, two sets of six ghost-interface planes, the node-type
+ * map and the boundary-condition class table. Every call returns 0 on
+ * success and a negative MRLBM_E* code on failure; mrlbm_last_error() gives
+ * the text of the last failure on the calling thread. Nothing calls exit().
+ * A context is thread-compatible (not thread-safe); all device work of a
+ * context is ordered on its own HIP stream.
+ *
+ * Host arrays crossing this boundary are plain x-fastest arrays
+ * (index = x + NX*(y + NY*z)), as written by the reference's saveMacr
+ * (src/saveData.cu:7-260) and read by its bc_initialization loop
+ * (src/lbmInitialization.cu:958-975).
+ */
+#ifndef MRLBM_H
+#define MRLBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define MRLBM_OK          0
+#define MRLBM_EINVAL     -1  /* bad argument or unsupported parameter set */
+#define MRLBM_EHIP       -2  /* HIP runtime error */
+#define MRLBM_ENOMEM     -3  /* device allocation failed / over budget */
+#define MRLBM_EIO        -4  /* file error */
+#define MRLBM_ESTATE     -5  /* call not valid in the current state */
+#define MRLBM_ENUMERIC   -6  /* singular boundary system, NaN detected */
+
+/* moment storage precision (arithmetic is always FP32) */
+#define MRLBM_STORE_FP32  0
+#define MRLBM_STORE_FP16  1
+#define MRLBM_STORE_BF16  2
+
+/* second-moment initialisation */
+#define MRLBM_INIT_EQUILIBRIUM 0 /* m_ab = u_a u_b                                  */
+#define MRLBM_INIT_REFERENCE   1 /* m_ab from f_eq(RHO_0,u)/rho - cs2 d_ab, as in
+                                    src/lbmInitialization.cu:102-130              */
+
+/* step kernel family */
+#define MRLBM_KERNEL_AUTO   0
+#define MRLBM_KERNEL_TILE   1 /* one node per thread, 3-D tile, LDS streaming   */
+#define MRLBM_KERNEL_COLUMN 2 /* one (x,y) column per thread marching in z      */
+
+typedef struct mrlbm_ctx mrlbm_ctx;
+
+/* Lattice parameters. Replaces the compile-time constants of a case's
+ * constants.inc/model.inc (reference src/var.h:184-186, src/definitions.h). */
+typedef struct {
+    int nx, ny, nz;        /* NX, NY, NZ                                        */
+    int q;                 /* 19 (D3Q19) or 27 (D3Q27)                          */
+    int bx, by, bz;        /* tile shape; 0,0,0 = chosen by the library         */
+    int storage;           /* MRLBM_STORE_*                                     */
+    int wall_x, wall_y, wall_z; /* 0 periodic (BC_*_PERIODIC), 1 wall (BC_*_WALL) */
+    int kernel;            /* MRLBM_KERNEL_*                                    */
+    double tau;            /* TAU; omega = 1/TAU (src/definitions.h:8)          */
+    double rho0;           /* RHO_0                                             */
+    double fx, fy, fz;     /* body force L_Fx/L_Fy/L_Fz (src/mlbm.cu:76-78)     */
+    int device;            /* HIP device ordinal                                */
+} mrlbm_params;
+
+typedef struct {
+    int nx, ny, nz, q, bx, by, bz, storage, kernel;
+    int64_t step;             /* steps taken since initialisation/resume     */
+    int n_bc_classes;         /* distinct boundary classes in the node map   */
+    int has_solid;            /* any SOLID_NODE present                      */
+    double bytes_per_lu;      /* algorithmic HBM bytes per lattice update    */
+    double device_bytes;      /* HBM held by the context                     */
+} mrlbm_info;
+
+/* Last error text of the calling thread ("" if none). */
+const char* mrlbm_last_error(void);
+
+/* Allocate a lattice on the device.
+ * Replaces allocateDeviceMemory/interfaceMalloc (src/main.cuh:874, :455). */
+int mrlbm_create(const mrlbm_params* p, mrlbm_ctx** out);
+
+/* Release everything (src/main.cuh:74 interfaceFree). NULL is accepted. */
+int mrlbm_destroy(mrlbm_ctx* ctx);
+
+/* Wall-velocity table entry k (0..3), unscaled lattice units.
+ * Replaces UX_BC/UY_BC/UZ_BC[4] of a case's constants.inc. Call before
+ * mrlbm_set_node_types. */
+int mrlbm_set_bc_velocity(mrlbm_ctx* ctx, int k, double ux, double uy, double uz);
+
+/* Node-type map, x-fastest, reference encoding (include/mrlbm_nodetype.h).
+ * Replaces hostInitialization_nodeType + its H2D copy
+ * (src/lbmInitialization.cu:958-975, src/main.cuh:996-1008).
+ * NULL = every node BULK (fully periodic lattice). */
+int mrlbm_set_node_types(mrlbm_ctx* ctx, const uint32_t* node_types);
+
+/* Initial macroscopic fields (x-fastest, unscaled): moments and both ghost
+ * sets. Replaces gpuInitialization_mom + gpuInitialization_pop
+ * (src/lbmInitialization.cu:28-218, :220-335). Resets the step counter. */
+int mrlbm_init_fields(mrlbm_ctx* ctx, const double* rho, const double* ux,
+                      const double* uy, const double* uz, int second_moment_mode);
+
+/* Built-in initial states evaluated in FP64 on the host (no N-sized caller
+ * arrays): uniform state, and the Taylor-Green vortex of
+ * cases/taylorGreen/flow_initialization.inc:1-4 with k = 2*pi/NX. */
+int mrlbm_init_uniform(mrlbm_ctx* ctx, double rho, double ux, double uy, double uz,
+                       int second_moment_mode);
+int mrlbm_init_taylor_green(mrlbm_ctx* ctx, double u0, int second_moment_mode);
+
+/* Advance nsteps time steps (stream-ordered, asynchronous).
+ * Replaces the step loop body gpuMomCollisionStream + swapGhostInterfaces
+ * (src/main.cu:223-230, src/mlbm.cu:3-714); no host sync per step. */
+int mrlbm_step(mrlbm_ctx* ctx, int nsteps);
+
+/* Advance nsteps steps bracketed by HIP events on the context stream;
+ * *elapsed_ms = event time (src/main.cuh:56-67 recordElapsedTime). */
+int mrlbm_time_steps(mrlbm_ctx* ctx, int nsteps, double* elapsed_ms);
+
+/* Wait for all work of the context. */
+int mrlbm_synchronize(mrlbm_ctx* ctx);
+
+/* Stored moments as doubles, x-fastest, 10 planes of N values in the
+ * reference order and scaling: rho-RHO_0, 3ux, 3uy, 3uz, 4.5mxx, 9mxy,
+ * 9mxz, 4.5myy, 9myz, 4.5mzz (src/arrayIndex.h:8-17,186). */
+int mrlbm_read_moments(mrlbm_ctx* ctx, double* out10n);
+
+/* Overwrite the stored moments (same format) and rebuild both ghost sets
+ * from them, as gpuInitialization_pop does. */
+int mrlbm_write_moments(mrlbm_ctx* ctx, const double* in10n);
+
+/* FP64 reductions over all nodes (solid included), divided by N:
+ * mean(rho - RHO_0) and mean(rho |u|^2 / 2).
+ * Replaces mean_rho / totalKineticEnergy (src/auxFunctions.cu:8-67,
+ * src/treatData.cu:105-165, src/reduction.cu:99-155). */
+int mrlbm_reduce(mrlbm_ctx* ctx, double* mean_mass, double* mean_ke);
+
+/* Checkpoint = moments + current ghost set + step + parameters, raw
+ * little-endian, versioned header. Replaces saveSimCheckpoint /
+ * loadSimCheckpoint (src/checkpoint.cu:372-382, :353-369). */
+int mrlbm_checkpoint_save(mrlbm_ctx* ctx, const char* path);
+int mrlbm_checkpoint_load(mrlbm_ctx* ctx, const char* path);
+
+/* Legacy-VTK STRUCTURED_POINTS file (big-endian float rho, velocity),
+ * as saveVarVTK (src/saveData.cu:303-406). */
+int mrlbm_save_vtk(mrlbm_ctx* ctx, const char* path);
+
+int mrlbm_get_info(mrlbm_ctx* ctx, mrlbm_info* out);
+
+/* Host-only diagnostic (no device work): the known-population mask and the
+ * 7x7 inverse moment-matching matrix the kernels use for a boundary node of
+ * type `node_type` (relaxation omega, wall velocity u). The reference has no
+ * counterpart: its case files hold the closed-form solutions of this system
+ * (e.g. cases/lidDrivenCavity_3D/bc_definition.inc). */
+int mrlbm_bc_class_info(int q, uint32_t node_type, double omega, const double u[3], uint32_t* known,
+                        double* ainv49);
+
+/* Measurement utility (no reference counterpart): time `iters` plain
+ * device-to-device copies of `bytes` on `device`; *ms_per_copy = mean event
+ * time of one copy (it reads and writes `bytes` each). */
+int mrlbm_copy_bench(int device, size_t bytes, int iters, double* ms_per_copy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MRLBM_H */

@@ -1,0 +1,618 @@
+// C ABI of the library (include/mrlbm.h): context ownership, node map,
+// initial state, step loop, read-back, reductions, checkpoint and VTK.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mrlbm.h"
+#include "bc_classes.h"
+#include "kernels.h"
+
+using namespace mrlbm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(MRLBM_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+size_t storage_bytes(int storage) { return storage == MRLBM_STORE_FP32 ? 4 : 2; }
+
+}  // namespace
+
+struct mrlbm_ctx {
+    mrlbm_params p;
+    int bx, by, bz, T, ntx, nty, ntz, ntiles, qf;
+    size_t N;
+    hipStream_t stream = nullptr;
+    void* d_mom = nullptr;
+    uint8_t* d_cls = nullptr;
+    float* d_bc = nullptr;
+    float* d_ghost[2][6] = {};
+    size_t face_elems[6] = {};
+    int cur = 0;  // ghost set read by the next step
+    int64_t step = 0;
+    bool has_bc = false, has_solid = false, initialised = false;
+    int n_classes = 0;
+    double bc_vel[4][3] = {};
+    double* d_partial = nullptr;
+    double* d_red = nullptr;
+    int red_blocks = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double device_bytes = 0;
+
+    size_t mom_index(int x, int y, int z, int m) const {
+        const int tile = (x / bx) + ntx * ((y / by) + nty * (z / bz));
+        const int t = (x % bx) + bx * ((y % by) + by * (z % bz));
+        return (size_t)tile * 10 * T + (size_t)m * T + t;
+    }
+    size_t node_index(int x, int y, int z) const {
+        const int tile = (x / bx) + ntx * ((y / by) + nty * (z / bz));
+        const int t = (x % bx) + bx * ((y % by) + by * (z % bz));
+        return (size_t)tile * T + t;
+    }
+    StepArgs args() const {
+        StepArgs a{};
+        a.mom = d_mom;
+        a.cls = d_cls;
+        a.bc = d_bc;
+        for (int f = 0; f < 6; f++) {
+            a.gin[f] = d_ghost[cur][f];
+            a.gout[f] = d_ghost[1 - cur][f];
+        }
+        a.ntx = ntx;
+        a.nty = nty;
+        a.ntz = ntz;
+        a.nx = p.nx;
+        a.ny = p.ny;
+        a.nz = p.nz;
+        a.wall_x = p.wall_x;
+        a.wall_y = p.wall_y;
+        a.wall_z = p.wall_z;
+        a.rho0 = (float)p.rho0;
+        const double omega = 1.0 / p.tau;
+        // relaxation constants as src/mlbm.cu:56-60 derives them
+        const float om = (float)omega;
+        a.R.omega = om;
+        a.R.t_omega = 1.0f - om;
+        a.R.tt_omega = 1.0f - om / 2.0f;
+        a.R.omega_d2 = om / 2.0f;
+        a.R.tt_omega_t3 = a.R.tt_omega * 3.0f;
+        a.R.fx = (float)p.fx;
+        a.R.fy = (float)p.fy;
+        a.R.fz = (float)p.fz;
+        for (int m = 0; m < 10; m++) a.mscale[m] = a.mrscale[m] = 1.0f;
+        return a;
+    }
+    double bytes_per_lu() const {
+        const double sm = (double)storage_bytes(p.storage);
+        const double sg = 4.0;
+        double b = 2.0 * 10.0 * sm + 4.0 * qf * sg * (1.0 / bx + 1.0 / by + 1.0 / bz);
+        if (has_bc) b += 1.0;
+        return b;
+    }
+};
+
+extern "C" {
+
+const char* mrlbm_last_error(void) { return g_err.c_str(); }
+
+int mrlbm_destroy(mrlbm_ctx* c) {
+    if (!c) return MRLBM_OK;
+    hipSetDevice(c->p.device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    hipFree(c->d_mom);
+    hipFree(c->d_cls);
+    hipFree(c->d_bc);
+    for (int s = 0; s < 2; s++)
+        for (int f = 0; f < 6; f++) hipFree(c->d_ghost[s][f]);
+    hipFree(c->d_partial);
+    hipFree(c->d_red);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return MRLBM_OK;
+}
+
+int mrlbm_create(const mrlbm_params* pp, mrlbm_ctx** out) {
+    if (!pp || !out) return fail(MRLBM_EINVAL, "null argument");
+    *out = nullptr;
+    const mrlbm_params& p = *pp;
+    if (p.q != 19 && p.q != 27) return fail(MRLBM_EINVAL, "q must be 19 or 27 (got %d)", p.q);
+    if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0) return fail(MRLBM_EINVAL, "lattice extents must be positive");
+    if (!(p.tau > 0.5)) return fail(MRLBM_EINVAL, "tau must exceed 0.5 (got %g)", p.tau);
+    if (p.storage != MRLBM_STORE_FP32 && p.storage != MRLBM_STORE_FP16 && p.storage != MRLBM_STORE_BF16)
+        return fail(MRLBM_EINVAL, "unknown storage precision %d", p.storage);
+    if (p.kernel != MRLBM_KERNEL_AUTO && p.kernel != MRLBM_KERNEL_TILE && p.kernel != MRLBM_KERNEL_COLUMN)
+        return fail(MRLBM_EINVAL, "unknown kernel family %d", p.kernel);
+    int bx = p.bx, by = p.by, bz = p.bz;
+    if (bx == 0 && by == 0 && bz == 0) bx = by = bz = 8;
+    if (!tile_shape_supported(p.q, bx, by, bz, p.storage))
+        return fail(MRLBM_EINVAL, "tile shape %dx%dx%d not built for q=%d storage=%d", bx, by, bz, p.q, p.storage);
+    if (p.nx % bx || p.ny % by || p.nz % bz)
+        return fail(MRLBM_EINVAL, "lattice %dx%dx%d is not a multiple of the tile %dx%dx%d", p.nx, p.ny, p.nz, bx, by,
+                    bz);
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (p.device < 0 || p.device >= ndev) return fail(MRLBM_EINVAL, "device %d not present", p.device);
+    HIPCHK(hipSetDevice(p.device));
+
+    mrlbm_ctx* c = new mrlbm_ctx();
+    c->p = p;
+    if (c->p.rho0 == 0.0) c->p.rho0 = 1.0;
+    c->p.bx = c->bx = bx;
+    c->p.by = c->by = by;
+    c->p.bz = c->bz = bz;
+    c->p.kernel = MRLBM_KERNEL_TILE;
+    c->T = bx * by * bz;
+    c->ntx = p.nx / bx;
+    c->nty = p.ny / by;
+    c->ntz = p.nz / bz;
+    c->ntiles = c->ntx * c->nty * c->ntz;
+    c->qf = p.q == 19 ? 5 : 9;
+    c->N = (size_t)p.nx * p.ny * p.nz;
+
+    auto alloc = [&](void** ptr, size_t bytes) -> int {
+        hipError_t e = hipMalloc(ptr, bytes);
+        if (e != hipSuccess) {
+            *ptr = nullptr;
+            return fail(MRLBM_ENOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+        }
+        c->device_bytes += (double)bytes;
+        return MRLBM_OK;
+    };
+    int rc;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        mrlbm_destroy(c);
+        return fail(MRLBM_EHIP, "hipStreamCreate failed");
+    }
+    if ((rc = alloc(&c->d_mom, c->N * 10 * storage_bytes(p.storage)))) {
+        mrlbm_destroy(c);
+        return rc;
+    }
+    c->face_elems[0] = c->face_elems[1] = (size_t)by * bz * c->qf * c->ntiles;
+    c->face_elems[2] = c->face_elems[3] = (size_t)bx * bz * c->qf * c->ntiles;
+    c->face_elems[4] = c->face_elems[5] = (size_t)bx * by * c->qf * c->ntiles;
+    for (int s = 0; s < 2; s++)
+        for (int f = 0; f < 6; f++)
+            if ((rc = alloc((void**)&c->d_ghost[s][f], c->face_elems[f] * sizeof(float)))) {
+                mrlbm_destroy(c);
+                return rc;
+            }
+    c->red_blocks = 1024;
+    if ((rc = alloc((void**)&c->d_partial, sizeof(double) * 2 * c->red_blocks)) ||
+        (rc = alloc((void**)&c->d_red, sizeof(double) * 2))) {
+        mrlbm_destroy(c);
+        return rc;
+    }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        mrlbm_destroy(c);
+        return fail(MRLBM_EHIP, "hipEventCreate failed");
+    }
+    *out = c;
+    return MRLBM_OK;
+}
+
+int mrlbm_set_bc_velocity(mrlbm_ctx* c, int k, double ux, double uy, double uz) {
+    if (!c) return fail(MRLBM_EINVAL, "null context");
+    if (k < 0 || k > 3) return fail(MRLBM_EINVAL, "wall-velocity index %d outside 0..3", k);
+    c->bc_vel[k][0] = ux;
+    c->bc_vel[k][1] = uy;
+    c->bc_vel[k][2] = uz;
+    return MRLBM_OK;
+}
+
+int mrlbm_set_node_types(mrlbm_ctx* c, const uint32_t* types) {
+    if (!c) return fail(MRLBM_EINVAL, "null context");
+    HIPCHK(hipSetDevice(c->p.device));
+    BcClassTable tab;
+    std::string err = build_class_table(c->p.q, 1.0 / c->p.tau, c->bc_vel, types, c->N, tab);
+    if (!err.empty()) return fail(MRLBM_EINVAL, "%s", err.c_str());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    hipFree(c->d_cls);
+    hipFree(c->d_bc);
+    c->d_cls = nullptr;
+    c->d_bc = nullptr;
+    c->has_bc = tab.has_bc;
+    c->has_solid = tab.has_solid;
+    c->n_classes = (int)tab.keys.size() - 1;
+    if (!c->has_bc) return MRLBM_OK;
+    std::vector<uint8_t> blk(c->N);
+    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) blk[c->node_index(x, y, z)] = tab.class_of[x + (size_t)nx * (y + (size_t)ny * z)];
+    HIPCHK(hipMalloc((void**)&c->d_cls, c->N));
+    HIPCHK(hipMalloc((void**)&c->d_bc, tab.table.size() * sizeof(float)));
+    HIPCHK(hipMemcpy(c->d_cls, blk.data(), c->N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_bc, tab.table.data(), tab.table.size() * sizeof(float), hipMemcpyHostToDevice));
+    return MRLBM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Stored (scaled) moments of one node from unscaled macroscopic fields.
+void node_moments(int q, int mode, double rho0, double rho, double ux, double uy, double uz, double m[10]) {
+    double pxx, pxy, pxz, pyy, pyz, pzz;
+    if (mode == MRLBM_INIT_EQUILIBRIUM) {
+        pxx = ux * ux;
+        pxy = ux * uy;
+        pxz = ux * uz;
+        pyy = uy * uy;
+        pyz = uy * uz;
+        pzz = uz * uz;
+    } else {
+        // f_eq(w_i RHO_0, u), summed over the D3Q19 index subsets and divided
+        // by rho, as src/lbmInitialization.cu:102-130 does for either lattice.
+        static const int CX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0};
+        static const int CY[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1};
+        static const int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1};
+        double f[19];
+        const double uu = ux * ux + uy * uy + uz * uz;
+        for (int i = 0; i < 19; i++) {
+            const int cls = i == 0 ? 0 : (i < 7 ? 1 : 2);
+            const double w = q == 19 ? (cls == 0 ? 1.0 / 3 : (cls == 1 ? 1.0 / 18 : 1.0 / 36))
+                                     : (cls == 0 ? 8.0 / 27 : (cls == 1 ? 2.0 / 27 : 1.0 / 54));
+            const double uc3 = 3.0 * (ux * CX[i] + uy * CY[i] + uz * CZ[i]);
+            f[i] = (w * rho0) * ((1.0 - 1.5 * uu) + uc3 * (1.0 + uc3 * 0.5));
+        }
+        const double ir = 1.0 / rho, cs2 = 1.0 / 3.0;
+        pxx = (f[1] + f[2] + f[7] + f[8] + f[9] + f[10] + f[13] + f[14] + f[15] + f[16]) * ir - cs2;
+        pxy = ((f[7] + f[8]) - (f[13] + f[14])) * ir;
+        pxz = ((f[9] + f[10]) - (f[15] + f[16])) * ir;
+        pyy = (f[3] + f[4] + f[7] + f[8] + f[11] + f[12] + f[13] + f[14] + f[17] + f[18]) * ir - cs2;
+        pyz = ((f[11] + f[12]) - (f[17] + f[18])) * ir;
+        pzz = (f[5] + f[6] + f[9] + f[10] + f[11] + f[12] + f[15] + f[16] + f[17] + f[18]) * ir - cs2;
+    }
+    m[0] = rho - rho0;
+    m[1] = 3.0 * ux;
+    m[2] = 3.0 * uy;
+    m[3] = 3.0 * uz;
+    m[4] = 4.5 * pxx;
+    m[5] = 9.0 * pxy;
+    m[6] = 9.0 * pxz;
+    m[7] = 4.5 * pyy;
+    m[8] = 9.0 * pyz;
+    m[9] = 4.5 * pzz;
+}
+
+int upload_and_ghosts(mrlbm_ctx* c, const std::vector<float>& blk) {
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(c->d_mom, blk.data(), blk.size() * sizeof(float), hipMemcpyHostToDevice));
+    StepArgs a = c->args();
+    for (int s = 0; s < 2; s++) {
+        HIPCHK(launch_init_ghosts(c->p.q, c->bx, c->by, c->bz, c->p.storage, a, c->d_ghost[s], c->ntiles, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->cur = 0;
+    c->step = 0;
+    c->initialised = true;
+    return MRLBM_OK;
+}
+
+template <typename F>
+int init_with(mrlbm_ctx* c, int mode, F&& field) {
+    if (mode != MRLBM_INIT_EQUILIBRIUM && mode != MRLBM_INIT_REFERENCE)
+        return fail(MRLBM_EINVAL, "unknown second-moment mode %d", mode);
+    if (c->p.storage != MRLBM_STORE_FP32) return fail(MRLBM_EINVAL, "reduced-precision storage not built yet");
+    std::vector<float> blk(c->N * 10);
+    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                double r, ux, uy, uz, m[10];
+                field(x, y, z, r, ux, uy, uz);
+                if (!(r > 0)) return fail(MRLBM_EINVAL, "non-positive density at (%d,%d,%d)", x, y, z);
+                node_moments(c->p.q, mode, c->p.rho0, r, ux, uy, uz, m);
+                for (int k = 0; k < 10; k++) blk[c->mom_index(x, y, z, k)] = (float)m[k];
+            }
+    return upload_and_ghosts(c, blk);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mrlbm_init_fields(mrlbm_ctx* c, const double* rho, const double* ux, const double* uy, const double* uz,
+                      int mode) {
+    if (!c || !rho || !ux || !uy || !uz) return fail(MRLBM_EINVAL, "null argument");
+    const size_t nx = c->p.nx, ny = c->p.ny;
+    return init_with(c, mode, [&](int x, int y, int z, double& r, double& a, double& b, double& d) {
+        const size_t j = x + nx * (y + ny * z);
+        r = rho[j];
+        a = ux[j];
+        b = uy[j];
+        d = uz[j];
+    });
+}
+
+int mrlbm_init_uniform(mrlbm_ctx* c, double rho, double ux, double uy, double uz, int mode) {
+    if (!c) return fail(MRLBM_EINVAL, "null context");
+    return init_with(c, mode, [&](int, int, int, double& r, double& a, double& b, double& d) {
+        r = rho;
+        a = ux;
+        b = uy;
+        d = uz;
+    });
+}
+
+int mrlbm_init_taylor_green(mrlbm_ctx* c, double u0, int mode) {
+    if (!c) return fail(MRLBM_EINVAL, "null context");
+    // cases/taylorGreen/flow_initialization.inc:1-4 with L = N/(2 pi)
+    const double kx = 2.0 * M_PI / c->p.nx, ky = 2.0 * M_PI / c->p.ny, kz = 2.0 * M_PI / c->p.nz;
+    const double rho0 = c->p.rho0;
+    return init_with(c, mode, [&](int x, int y, int z, double& r, double& a, double& b, double& d) {
+        r = rho0 + (3.0 / 16.0) * rho0 * u0 * u0 * (std::cos(2 * kx * x) + std::cos(2 * ky * y)) *
+                       (std::cos(2 * kz * z) + 2.0);
+        a = u0 * std::sin(kx * x) * std::cos(ky * y) * std::cos(kz * z);
+        b = -u0 * std::cos(kx * x) * std::sin(ky * y) * std::cos(kz * z);
+        d = 0.0;
+    });
+}
+
+static int do_steps(mrlbm_ctx* c, int nsteps) {
+    for (int s = 0; s < nsteps; s++) {
+        StepArgs a = c->args();
+        hipError_t e =
+            launch_step_tile(c->p.q, c->bx, c->by, c->bz, c->has_bc, c->p.storage, a, c->ntiles, c->stream);
+        if (e != hipSuccess) return fail(MRLBM_EHIP, "step launch failed: %s", hipGetErrorString(e));
+        c->cur = 1 - c->cur;
+        c->step++;
+    }
+    return MRLBM_OK;
+}
+
+int mrlbm_step(mrlbm_ctx* c, int nsteps) {
+    if (!c) return fail(MRLBM_EINVAL, "null context");
+    if (!c->initialised) return fail(MRLBM_ESTATE, "lattice not initialised");
+    if (nsteps < 0) return fail(MRLBM_EINVAL, "negative step count");
+    HIPCHK(hipSetDevice(c->p.device));
+    return do_steps(c, nsteps);
+}
+
+int mrlbm_time_steps(mrlbm_ctx* c, int nsteps, double* ms) {
+    if (!c || !ms) return fail(MRLBM_EINVAL, "null argument");
+    if (!c->initialised) return fail(MRLBM_ESTATE, "lattice not initialised");
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    int rc = do_steps(c, nsteps);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float f = 0;
+    HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+    *ms = f;
+    return MRLBM_OK;
+}
+
+int mrlbm_synchronize(mrlbm_ctx* c) {
+    if (!c) return fail(MRLBM_EINVAL, "null context");
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MRLBM_OK;
+}
+
+int mrlbm_read_moments(mrlbm_ctx* c, double* out) {
+    if (!c || !out) return fail(MRLBM_EINVAL, "null argument");
+    if (c->p.storage != MRLBM_STORE_FP32) return fail(MRLBM_EINVAL, "reduced-precision storage not built yet");
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<float> blk(c->N * 10);
+    HIPCHK(hipMemcpy(blk.data(), c->d_mom, blk.size() * sizeof(float), hipMemcpyDeviceToHost));
+    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    for (int k = 0; k < 10; k++)
+        for (int z = 0; z < nz; z++)
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++)
+                    out[(size_t)k * c->N + x + (size_t)nx * (y + (size_t)ny * z)] = blk[c->mom_index(x, y, z, k)];
+    return MRLBM_OK;
+}
+
+int mrlbm_write_moments(mrlbm_ctx* c, const double* in) {
+    if (!c || !in) return fail(MRLBM_EINVAL, "null argument");
+    if (c->p.storage != MRLBM_STORE_FP32) return fail(MRLBM_EINVAL, "reduced-precision storage not built yet");
+    std::vector<float> blk(c->N * 10);
+    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    for (int k = 0; k < 10; k++)
+        for (int z = 0; z < nz; z++)
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++)
+                    blk[c->mom_index(x, y, z, k)] = (float)in[(size_t)k * c->N + x + (size_t)nx * (y + (size_t)ny * z)];
+    return upload_and_ghosts(c, blk);
+}
+
+int mrlbm_reduce(mrlbm_ctx* c, double* mass, double* ke) {
+    if (!c || !mass || !ke) return fail(MRLBM_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(launch_reduce(c->p.storage, c->d_mom, c->ntiles, c->T, (float)c->p.rho0, c->d_partial, c->red_blocks,
+                         c->d_red, c->stream));
+    double h[2];
+    HIPCHK(hipMemcpyAsync(h, c->d_red, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *mass = h[0] / (double)c->N;
+    *ke = h[1] / (double)c->N;
+    return MRLBM_OK;
+}
+
+int mrlbm_get_info(mrlbm_ctx* c, mrlbm_info* o) {
+    if (!c || !o) return fail(MRLBM_EINVAL, "null argument");
+    o->nx = c->p.nx;
+    o->ny = c->p.ny;
+    o->nz = c->p.nz;
+    o->q = c->p.q;
+    o->bx = c->bx;
+    o->by = c->by;
+    o->bz = c->bz;
+    o->storage = c->p.storage;
+    o->kernel = c->p.kernel;
+    o->step = c->step;
+    o->n_bc_classes = c->n_classes;
+    o->has_solid = c->has_solid;
+    o->bytes_per_lu = c->bytes_per_lu();
+    o->device_bytes = c->device_bytes;
+    return MRLBM_OK;
+}
+
+int mrlbm_bc_class_info(int q, uint32_t node_type, double omega, const double u[3], uint32_t* known,
+                        double* ainv49) {
+    if (!u || !known || !ainv49) return fail(MRLBM_EINVAL, "null argument");
+    if (q != 19 && q != 27) return fail(MRLBM_EINVAL, "q must be 19 or 27");
+    *known = known_mask(q, node_type & 0xFFu);
+    if (!build_bc_inverse(q, *known, omega, u, ainv49)) return fail(MRLBM_ENUMERIC, "singular boundary system");
+    return MRLBM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// checkpoint: header, then moments (block layout, storage type), then the
+// six faces of the ghost set the next step reads.
+
+struct CkHeader {
+    char magic[8];
+    uint32_t version;
+    int32_t nx, ny, nz, q, bx, by, bz, storage, wall_x, wall_y, wall_z, pad;
+    int64_t step;
+    double tau, rho0, fx, fy, fz;
+    uint64_t mom_bytes, face_bytes[6];
+};
+
+int mrlbm_checkpoint_save(mrlbm_ctx* c, const char* path) {
+    if (!c || !path) return fail(MRLBM_EINVAL, "null argument");
+    if (!c->initialised) return fail(MRLBM_ESTATE, "lattice not initialised");
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    CkHeader h{};
+    std::memcpy(h.magic, "MRLBMCK1", 8);
+    h.version = 1;
+    h.nx = c->p.nx;
+    h.ny = c->p.ny;
+    h.nz = c->p.nz;
+    h.q = c->p.q;
+    h.bx = c->bx;
+    h.by = c->by;
+    h.bz = c->bz;
+    h.storage = c->p.storage;
+    h.wall_x = c->p.wall_x;
+    h.wall_y = c->p.wall_y;
+    h.wall_z = c->p.wall_z;
+    h.step = c->step;
+    h.tau = c->p.tau;
+    h.rho0 = c->p.rho0;
+    h.fx = c->p.fx;
+    h.fy = c->p.fy;
+    h.fz = c->p.fz;
+    h.mom_bytes = c->N * 10 * storage_bytes(c->p.storage);
+    for (int f = 0; f < 6; f++) h.face_bytes[f] = c->face_elems[f] * sizeof(float);
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return fail(MRLBM_EIO, "cannot open '%s' for writing", path);
+    std::vector<char> buf(h.mom_bytes);
+    bool ok = std::fwrite(&h, sizeof h, 1, fp) == 1;
+    if (ok && hipMemcpy(buf.data(), c->d_mom, h.mom_bytes, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
+    if (ok) ok = std::fwrite(buf.data(), 1, h.mom_bytes, fp) == h.mom_bytes;
+    for (int f = 0; f < 6 && ok; f++) {
+        buf.resize(h.face_bytes[f]);
+        if (hipMemcpy(buf.data(), c->d_ghost[c->cur][f], h.face_bytes[f], hipMemcpyDeviceToHost) != hipSuccess)
+            ok = false;
+        if (ok) ok = std::fwrite(buf.data(), 1, h.face_bytes[f], fp) == h.face_bytes[f];
+    }
+    if (std::fclose(fp) != 0) ok = false;
+    if (!ok) return fail(MRLBM_EIO, "writing checkpoint '%s' failed", path);
+    return MRLBM_OK;
+}
+
+int mrlbm_checkpoint_load(mrlbm_ctx* c, const char* path) {
+    if (!c || !path) return fail(MRLBM_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return fail(MRLBM_EIO, "cannot open '%s'", path);
+    CkHeader h{};
+    if (std::fread(&h, sizeof h, 1, fp) != 1 || std::memcmp(h.magic, "MRLBMCK1", 8) != 0 || h.version != 1) {
+        std::fclose(fp);
+        return fail(MRLBM_EIO, "'%s' is not a version-1 checkpoint", path);
+    }
+    if (h.nx != c->p.nx || h.ny != c->p.ny || h.nz != c->p.nz || h.q != c->p.q || h.bx != c->bx || h.by != c->by ||
+        h.bz != c->bz || h.storage != c->p.storage) {
+        std::fclose(fp);
+        return fail(MRLBM_EINVAL, "checkpoint lattice/tile/storage does not match the context");
+    }
+    std::vector<char> buf(h.mom_bytes);
+    bool ok = std::fread(buf.data(), 1, h.mom_bytes, fp) == h.mom_bytes;
+    if (ok && hipMemcpy(c->d_mom, buf.data(), h.mom_bytes, hipMemcpyHostToDevice) != hipSuccess) ok = false;
+    for (int f = 0; f < 6 && ok; f++) {
+        if (h.face_bytes[f] != c->face_elems[f] * sizeof(float)) {
+            ok = false;
+            break;
+        }
+        buf.resize(h.face_bytes[f]);
+        ok = std::fread(buf.data(), 1, h.face_bytes[f], fp) == h.face_bytes[f];
+        // both sets: the one read next and the one written next (boundary
+        // slots that are never rewritten keep their saved contents)
+        for (int s = 0; s < 2 && ok; s++)
+            if (hipMemcpy(c->d_ghost[s][f], buf.data(), h.face_bytes[f], hipMemcpyHostToDevice) != hipSuccess)
+                ok = false;
+    }
+    std::fclose(fp);
+    if (!ok) return fail(MRLBM_EIO, "reading checkpoint '%s' failed", path);
+    c->cur = 0;
+    c->step = h.step;
+    c->initialised = true;
+    return MRLBM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// legacy VTK, big-endian float (src/saveData.cu:303-406)
+
+static void put_be(FILE* fp, float v) {
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    u = (u >> 24) | ((u >> 8) & 0xFF00u) | ((u << 8) & 0xFF0000u) | (u << 24);
+    std::fwrite(&u, 4, 1, fp);
+}
+
+int mrlbm_save_vtk(mrlbm_ctx* c, const char* path) {
+    if (!c || !path) return fail(MRLBM_EINVAL, "null argument");
+    std::vector<double> m(c->N * 10);
+    int rc = mrlbm_read_moments(c, m.data());
+    if (rc) return rc;
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return fail(MRLBM_EIO, "cannot open '%s' for writing", path);
+    std::fprintf(fp,
+                 "# vtk DataFile Version 3.0\nLBM output (binary)\nBINARY\nDATASET STRUCTURED_POINTS\n"
+                 "DIMENSIONS %d %d %d\nORIGIN 0 0 0\nSPACING 1 1 1\nPOINT_DATA %zu\n",
+                 c->p.nx, c->p.ny, c->p.nz, c->N);
+    std::fprintf(fp, "SCALARS rho float 1\nLOOKUP_TABLE default\n");
+    for (size_t j = 0; j < c->N; j++) put_be(fp, (float)(c->p.rho0 + m[j]));
+    std::fprintf(fp, "VECTORS velocity float\n");
+    for (size_t j = 0; j < c->N; j++) {
+        put_be(fp, (float)(m[c->N + j] / 3.0));
+        put_be(fp, (float)(m[2 * c->N + j] / 3.0));
+        put_be(fp, (float)(m[3 * c->N + j] / 3.0));
+    }
+    if (std::fclose(fp) != 0) return fail(MRLBM_EIO, "writing '%s' failed", path);
+    return MRLBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,36 @@
+// Boundary-condition classes: host-side construction of the per-class
+// moment-matching system used by bc_solve (lattice.h).
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace mrlbm {
+
+// Known-population mask of a node from the octant byte of its node type
+// (include/mrlbm_nodetype.h): population i is unknown iff every octant its
+// source direction -c_i touches is solid.
+uint32_t known_mask(int q, uint32_t octants);
+
+// Build the 7x7 system A for known set `known`, relaxation omega and wall
+// velocity u (unscaled), in FP64, and invert it. Returns false if singular.
+// Unknown vector x = (rho, P_xx, P_xy, P_xz, P_yy, P_yz, P_zz), P = rho m;
+// right-hand side rho_I (1, m_I).
+bool build_bc_inverse(int q, uint32_t known, double omega, const double u[3], double ainv[49], double* cond = nullptr);
+
+struct BcClassTable {
+    std::vector<uint32_t> keys;      // node-type key (octants | kind | velocity index) per class, index 1..
+    std::vector<float> table;        // BC_STRIDE floats per class (class 0 unused)
+    std::vector<uint8_t> class_of;   // per node in the caller's x-fastest order
+    bool has_bc = false, has_solid = false;
+};
+
+// Map a reference-encoded node-type map to class ids (0 bulk, 255 solid,
+// 1..254 boundary classes) and build the device table. bc_vel is the
+// UX_BC/UY_BC/UZ_BC table (4 x 3). Returns an empty string on success.
+std::string build_class_table(int q, double omega, const double bc_vel[4][3], const uint32_t* types, size_t n,
+                              BcClassTable& out);
+
+}  // namespace mrlbm

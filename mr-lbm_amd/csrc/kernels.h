@@ -1,0 +1,73 @@
+// Device-side argument blocks and launcher declarations shared by the
+// step kernels and the host library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lattice.h"
+
+namespace mrlbm {
+
+// Storage codecs. Arithmetic is always FP32; a codec only maps a stored
+// value to/from float. `enc` holds per-moment scale factors for the
+// reduced-precision codecs (power-of-two, so scaling is exact).
+template <int S> struct Codec;
+template <> struct Codec<0> {
+    using T = float;
+    __device__ __forceinline__ static float ld(const T* p, size_t i, float) { return p[i]; }
+    __device__ __forceinline__ static void st(T* p, size_t i, float v, float) { p[i] = v; }
+};
+
+// Arguments of one lattice step. Ghost faces: index 0 X_0, 1 X_1, 2 Y_0,
+// 3 Y_1, 4 Z_0, 5 Z_1. `gin` is read (previous step's outgoing
+// populations), `gout` is written; the host swaps them after each step as
+// swapGhostInterfaces does (src/main.cuh:353-363).
+struct StepArgs {
+    void* mom;               // 10 moment planes per tile, block-major SoA
+    const uint8_t* cls;      // node class per node (block layout) or null
+    const float* bc;         // boundary class table (BC_STRIDE floats each)
+    const float* gin[6];
+    float* gout[6];
+    int ntx, nty, ntz;       // tiles per axis
+    int nx, ny, nz;
+    int wall_x, wall_y, wall_z;
+    float rho0;
+    Relax R;
+    float mscale[10];        // storage scale per moment (reduced precision)
+    float mrscale[10];       // inverse scale
+};
+
+// Ghost-plane indices of the reference layout with runtime tile shape
+// (src/globalFunctions.h:45-124): X faces (ty,tz), Y faces (tx,tz),
+// Z faces (tx,ty); slot-major within a tile, tiles x-fastest.
+__host__ __device__ __forceinline__ size_t gidx_x(int ty, int tz, int slot, int tile, int BY, int BZ, int QF) {
+    return (size_t)ty + BY * ((size_t)tz + BZ * ((size_t)slot + (size_t)QF * tile));
+}
+__host__ __device__ __forceinline__ size_t gidx_y(int tx, int tz, int slot, int tile, int BX, int BZ, int QF) {
+    return (size_t)tx + BX * ((size_t)tz + BZ * ((size_t)slot + (size_t)QF * tile));
+}
+__host__ __device__ __forceinline__ size_t gidx_z(int tx, int ty, int slot, int tile, int BX, int BY, int QF) {
+    return (size_t)tx + BX * ((size_t)ty + BY * ((size_t)slot + (size_t)QF * tile));
+}
+
+struct GhostOut {
+    float* f[6];
+};
+
+// Launchers (step_tile.hip). Return hipError_t of the launch.
+hipError_t launch_step_tile(int q, int bx, int by, int bz, bool has_bc, int storage, const StepArgs& a,
+                            int ntiles, hipStream_t s);
+bool tile_shape_supported(int q, int bx, int by, int bz, int storage);
+
+// Rebuild all six ghost faces of buffer `g` from the stored moments,
+// unconditionally on every tile face (src/lbmInitialization.cu:220-335).
+hipError_t launch_init_ghosts(int q, int bx, int by, int bz, int storage, const StepArgs& a, float* const g[6],
+                              int ntiles, hipStream_t s);
+
+// FP64 partial sums of (rho - RHO_0) and rho |u|^2 / 2 per block; then one
+// block folds the partials. Deterministic (no atomics).
+hipError_t launch_reduce(int storage, const void* mom, size_t ntiles, int tile_nodes, float rho0, double* partial,
+                         int nblocks, double* out2, hipStream_t s);
+
+}  // namespace mrlbm

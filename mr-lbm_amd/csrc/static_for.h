@@ -1,0 +1,20 @@
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [0, N).
+// Used wherever a loop index selects a population register, so the index is
+// a constant expression and the population array never leaves registers.
+#pragma once
+
+#include <type_traits>
+#include <utility>
+
+namespace mrlbm {
+
+template <typename F, int... Is>
+__host__ __device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+__host__ __device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+}  // namespace mrlbm

@@ -1,0 +1,175 @@
+"""Host-side Python API of the MI355X moment-representation LBM library.
+
+A thin layer over the C ABI (include/mrlbm.h): `Lattice` owns one
+device-resident lattice; node-type maps use the reference's encoding
+(include/mrlbm_nodetype.h). Arrays crossing the boundary are numpy arrays
+of shape (nx, ny, nz) (x-fastest in memory when flattened in Fortran
+order) or flat x-fastest vectors.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (INIT_EQUILIBRIUM, INIT_REFERENCE, KERNEL_AUTO, KERNEL_COLUMN, KERNEL_TILE, STORE_BF16,
+                   STORE_FP16, STORE_FP32, MrlbmError)
+
+__all__ = ["Lattice", "MrlbmError", "STORE_FP32", "STORE_FP16", "STORE_BF16", "INIT_EQUILIBRIUM",
+           "INIT_REFERENCE", "KERNEL_AUTO", "KERNEL_TILE", "KERNEL_COLUMN", "MOMENT_NAMES"]
+
+MOMENT_NAMES = ("rho-rho0", "3ux", "3uy", "3uz", "4.5mxx", "9mxy", "9mxz", "4.5myy", "9myz", "4.5mzz")
+
+
+def _flat(a, n, dtype):
+    a = np.asarray(a)
+    if a.ndim == 3:
+        a = a.reshape(-1, order="F")
+    a = np.ascontiguousarray(a, dtype=dtype)
+    if a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+class Lattice:
+    """One lattice resident in HBM.
+
+    shape: (nx, ny, nz); q: 19 or 27; tau: relaxation time; walls: per-axis
+    0 periodic / 1 wall; force: body force (unscaled); tile: (bx, by, bz) or
+    None for the library default; storage: STORE_*.
+    """
+
+    def __init__(self, shape, q=19, tau=0.6, rho0=1.0, walls=(0, 0, 0), force=(0.0, 0.0, 0.0), tile=None,
+                 storage=STORE_FP32, kernel=KERNEL_AUTO, device=0):
+        L = _lib.load()
+        nx, ny, nz = shape
+        bx, by, bz = tile if tile else (0, 0, 0)
+        p = _lib.Params(nx, ny, nz, q, bx, by, bz, storage, walls[0], walls[1], walls[2], kernel, tau, rho0,
+                        force[0], force[1], force[2], device)
+        h = ctypes.c_void_p()
+        _lib.check(L.mrlbm_create(ctypes.byref(p), ctypes.byref(h)))
+        self._h = h
+        self.shape = (nx, ny, nz)
+        self.n = nx * ny * nz
+        self.q = q
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().mrlbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- set-up ---------------------------------------------------------
+    def set_bc_velocity(self, k, u):
+        _lib.check(_lib.load().mrlbm_set_bc_velocity(self._h, k, *map(float, u)))
+
+    def set_node_types(self, types):
+        L = _lib.load()
+        if types is None:
+            _lib.check(L.mrlbm_set_node_types(self._h, None))
+            return
+        t = _flat(types, self.n, np.uint32)
+        _lib.check(L.mrlbm_set_node_types(self._h, t.ctypes.data_as(_lib._u32p)))
+
+    def init_fields(self, rho, ux, uy, uz, mode=INIT_EQUILIBRIUM):
+        a = [_flat(v, self.n, np.float64) for v in (rho, ux, uy, uz)]
+        _lib.check(_lib.load().mrlbm_init_fields(self._h, *[v.ctypes.data_as(_lib._dp) for v in a], mode))
+
+    def init_uniform(self, rho=1.0, u=(0.0, 0.0, 0.0), mode=INIT_EQUILIBRIUM):
+        _lib.check(_lib.load().mrlbm_init_uniform(self._h, rho, *map(float, u), mode))
+
+    def init_taylor_green(self, u0, mode=INIT_EQUILIBRIUM):
+        _lib.check(_lib.load().mrlbm_init_taylor_green(self._h, u0, mode))
+
+    # -- stepping -------------------------------------------------------
+    def step(self, n=1):
+        _lib.check(_lib.load().mrlbm_step(self._h, n))
+
+    def time_steps(self, n):
+        """Run n steps between HIP events on the lattice's stream; returns ms."""
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().mrlbm_time_steps(self._h, n, ctypes.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        _lib.check(_lib.load().mrlbm_synchronize(self._h))
+
+    # -- read-back ------------------------------------------------------
+    def moments(self):
+        """(10, N) float64, x-fastest, stored convention (MOMENT_NAMES)."""
+        out = np.empty(10 * self.n)
+        _lib.check(_lib.load().mrlbm_read_moments(self._h, out.ctypes.data_as(_lib._dp)))
+        return out.reshape(10, self.n)
+
+    def write_moments(self, m):
+        m = np.ascontiguousarray(m, dtype=np.float64).reshape(-1)
+        if m.size != 10 * self.n:
+            raise ValueError("expected 10*N moments")
+        _lib.check(_lib.load().mrlbm_write_moments(self._h, m.ctypes.data_as(_lib._dp)))
+
+    def macroscopic(self, rho0=1.0):
+        """rho, ux, uy, uz as (nx, ny, nz) arrays (unscaled)."""
+        m = self.moments()
+        f = lambda v: v.reshape(self.shape, order="F")
+        return f(m[0] + rho0), f(m[1] / 3), f(m[2] / 3), f(m[3] / 3)
+
+    def reduce(self):
+        """(mean(rho - rho0), mean(rho |u|^2 / 2)) in FP64."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _lib.check(_lib.load().mrlbm_reduce(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def checkpoint_save(self, path):
+        _lib.check(_lib.load().mrlbm_checkpoint_save(self._h, str(path).encode()))
+
+    def checkpoint_load(self, path):
+        _lib.check(_lib.load().mrlbm_checkpoint_load(self._h, str(path).encode()))
+
+    def save_vtk(self, path):
+        _lib.check(_lib.load().mrlbm_save_vtk(self._h, str(path).encode()))
+
+    def info(self):
+        i = _lib.Info()
+        _lib.check(_lib.load().mrlbm_get_info(self._h, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in _lib.Info._fields_}
+
+
+def copy_bandwidth(nbytes=1 << 30, iters=20, device=0):
+    """Measured device copy bandwidth in GB/s (read + write bytes / time)."""
+    ms = ctypes.c_double()
+    _lib.check(_lib.load().mrlbm_copy_bench(device, nbytes, iters, ctypes.byref(ms)))
+    return 2.0 * nbytes / (ms.value * 1e-3) / 1e9, ms.value
+
+
+def bc_class_info(q, node_type, omega, u=(0.0, 0.0, 0.0)):
+    """(known mask, 7x7 inverse) the kernels use for a boundary node type (host only)."""
+    known = ctypes.c_uint32()
+    a = np.zeros(49)
+    uu = np.asarray(u, dtype=np.float64)
+    _lib.check(_lib.load().mrlbm_bc_class_info(q, int(node_type), float(omega), uu.ctypes.data_as(_lib._dp),
+                                               ctypes.byref(known), a.ctypes.data_as(_lib._dp)))
+    return known.value, a.reshape(7, 7)
+
+
+def bc_solve_host(q, node_type, omega, u, pop):
+    """FP64 evaluation of the kernels' boundary rule for one node (host, for
+    checks): unscaled rho, u, m as a bc_definition.inc fragment leaves them."""
+    known, ainv = bc_class_info(q, node_type, omega, u)
+    c = np.array([[0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1],
+                  [0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1],
+                  [0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1]])[:, :q]
+    k = np.array([(known >> i) & 1 for i in range(q)], bool)
+    f = np.where(k, pop, 0.0)
+    r = f.sum()
+    t = 1.0 / 3.0
+    b = np.array([1.0, (f * c[0] * c[0]).sum() / r - t, (f * c[0] * c[1]).sum() / r, (f * c[0] * c[2]).sum() / r,
+                  (f * c[1] * c[1]).sum() / r - t, (f * c[1] * c[2]).sum() / r, (f * c[2] * c[2]).sum() / r - t])
+    y = ainv @ b
+    return np.concatenate([[r * y[0]], np.asarray(u, float), y[1:] / y[0]])

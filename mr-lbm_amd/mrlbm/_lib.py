@@ -1,0 +1,97 @@
+"""ctypes binding of libmrlbm.so (include/mrlbm.h).
+
+The shared library is the product: every compute call below goes to the HIP
+kernels inside it. There is no fallback; if the library is missing, loading
+raises.
+"""
+import ctypes
+import os
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))  # mr-lbm_amd/
+LIB_PATH = os.path.join(PKG_ROOT, "libmrlbm.so")
+
+STORE_FP32, STORE_FP16, STORE_BF16 = 0, 1, 2
+INIT_EQUILIBRIUM, INIT_REFERENCE = 0, 1
+KERNEL_AUTO, KERNEL_TILE, KERNEL_COLUMN = 0, 1, 2
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("q", ctypes.c_int),
+        ("bx", ctypes.c_int), ("by", ctypes.c_int), ("bz", ctypes.c_int), ("storage", ctypes.c_int),
+        ("wall_x", ctypes.c_int), ("wall_y", ctypes.c_int), ("wall_z", ctypes.c_int), ("kernel", ctypes.c_int),
+        ("tau", ctypes.c_double), ("rho0", ctypes.c_double),
+        ("fx", ctypes.c_double), ("fy", ctypes.c_double), ("fz", ctypes.c_double),
+        ("device", ctypes.c_int),
+    ]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("q", ctypes.c_int),
+        ("bx", ctypes.c_int), ("by", ctypes.c_int), ("bz", ctypes.c_int), ("storage", ctypes.c_int),
+        ("kernel", ctypes.c_int), ("step", ctypes.c_int64), ("n_bc_classes", ctypes.c_int),
+        ("has_solid", ctypes.c_int), ("bytes_per_lu", ctypes.c_double), ("device_bytes", ctypes.c_double),
+    ]
+
+
+class MrlbmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mrlbm error {code}: {msg}")
+        self.code = code
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+_lib = None
+
+# name -> argtypes; every entry returns int status unless listed in _RESTYPES
+_SIGS = {
+    "mrlbm_create": [ctypes.POINTER(Params), ctypes.POINTER(_vp)],
+    "mrlbm_destroy": [_vp],
+    "mrlbm_set_bc_velocity": [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+    "mrlbm_set_node_types": [_vp, _u32p],
+    "mrlbm_init_fields": [_vp, _dp, _dp, _dp, _dp, ctypes.c_int],
+    "mrlbm_init_uniform": [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int],
+    "mrlbm_init_taylor_green": [_vp, ctypes.c_double, ctypes.c_int],
+    "mrlbm_step": [_vp, ctypes.c_int],
+    "mrlbm_time_steps": [_vp, ctypes.c_int, _dp],
+    "mrlbm_synchronize": [_vp],
+    "mrlbm_read_moments": [_vp, _dp],
+    "mrlbm_write_moments": [_vp, _dp],
+    "mrlbm_reduce": [_vp, _dp, _dp],
+    "mrlbm_checkpoint_save": [_vp, ctypes.c_char_p],
+    "mrlbm_checkpoint_load": [_vp, ctypes.c_char_p],
+    "mrlbm_save_vtk": [_vp, ctypes.c_char_p],
+    "mrlbm_get_info": [_vp, ctypes.POINTER(Info)],
+    "mrlbm_last_error": [],
+    "mrlbm_bc_class_info": [ctypes.c_int, ctypes.c_uint32, ctypes.c_double, _dp, _u32p, _dp],
+    "mrlbm_copy_bench": [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _dp],
+}
+_RESTYPES = {"mrlbm_last_error": ctypes.c_char_p}
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def load():
+    """Load libmrlbm.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built: run `make -C mr-lbm_amd` or __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise MrlbmError(rc, load().mrlbm_last_error().decode())
+    return rc

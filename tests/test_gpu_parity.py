@@ -1,0 +1,194 @@
+"""HIP kernels (through the C ABI) vs the FP64 oracle on the same inputs.
+
+Bar: per-field relative L2 <= 1e-5 (FP32 arithmetic vs FP64) after the
+stated steps; bit-identical results where only the data movement differs
+(tile shapes, checkpoint/resume)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import mrlbm
+from fields import rel_l2
+from mrlbm import geometry
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def run_pair(shape, q, tau, steps, walls=(0, 0, 0), types=None, force=(0, 0, 0), init="tgv", u0=0.05,
+             bc_vel=None, tile=None):
+    say(f"case shape={shape} q={q} tau={tau} steps={steps} walls={walls} tile={tile} init={init}")
+    lat = mrlbm.Lattice(shape, q=q, tau=tau, walls=walls, force=force, tile=tile)
+    orc = O.Oracle(*shape, q=q, tau=tau, force=force, walls=walls)
+    if bc_vel is not None:
+        lat.set_bc_velocity(0, bc_vel)
+        orc.set_bc_velocity(0, bc_vel)
+    lat.set_node_types(types)
+    orc.set_node_types(types)
+    if init == "tgv":
+        lat.init_taylor_green(u0)
+        orc.init_taylor_green(u0)
+    else:
+        lat.init_uniform()
+        orc.init_uniform()
+    lat.step(steps)
+    orc.step(steps)
+    g = lat.moments()
+    o = orc.moments()
+    lat.close()
+    return g, o, orc
+
+
+@pytest.mark.parametrize("n,steps", [(32, 300), (64, 200)])
+def test_taylor_green_d3q19(n, steps):
+    g, o, _ = run_pair((n, n, n), 19, 0.53, steps)
+    e = rel_l2(g, o)
+    say("rel L2", e)
+    assert max(e.values()) <= TOL
+
+
+def test_taylor_green_d3q27():
+    g, o, _ = run_pair((32, 32, 32), 27, 0.56, 200)
+    e = rel_l2(g, o)
+    say("rel L2", e)
+    assert max(e.values()) <= TOL
+
+
+def test_taylor_green_non_cubic():
+    g, o, _ = run_pair((64, 32, 16), 19, 0.6, 150)
+    e = rel_l2(g, o)
+    say("rel L2", e)
+    assert max(e.values()) <= TOL
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_lid_driven_cavity(q):
+    n = 32
+    types = geometry.cavity(n, lid_axis=2)
+    u = 0.05
+    tau = 0.5 + 3 * u * n / 100.0
+    g, o, orc = run_pair((n, n, n), q, tau, 300, walls=(1, 1, 1), types=types, init="uniform", bc_vel=(u, 0, 0))
+    assert orc.count_nan() == 0
+    e = rel_l2(g, o)
+    say("rel L2", e)
+    assert max(e.values()) <= TOL
+
+
+def test_channel_with_spheres_and_force():
+    shape = (48, 32, 32)
+    solid = geometry.spheres(shape, [(12, 16, 15), (32, 9, 20)], [5, 4])
+    types = geometry.classify(shape, (0, 1, 1), solid)
+    g, o, orc = run_pair(shape, 19, 0.62, 300, walls=(0, 1, 1), types=types, force=(2e-5, 0, 0), init="uniform")
+    assert orc.count_nan() == 0
+    fluid = np.asarray(solid, bool).reshape(-1, order="F") == 0
+    e = rel_l2(g[:, fluid], o[:, fluid])
+    say("rel L2", e)
+    assert max(e.values()) <= TOL
+
+
+def test_tile_shapes_bit_identical():
+    n = 64
+    ref = None
+    for tile in [(8, 8, 8), (16, 8, 4), (32, 4, 4), (64, 4, 2)]:
+        say("tile", tile)
+        with mrlbm.Lattice((n, n, n), q=19, tau=0.55, tile=tile) as lat:
+            lat.init_taylor_green(0.05)
+            lat.step(60)
+            m = lat.moments()
+        if ref is None:
+            ref = m
+        else:
+            assert np.array_equal(m, ref), f"tile {tile} differs"
+
+
+def test_tile_shapes_bit_identical_with_walls():
+    n = 32
+    types = geometry.cavity(n)
+    ref = None
+    for tile in [(8, 8, 8), (32, 4, 4)]:
+        with mrlbm.Lattice((n, n, n), q=19, tau=0.6, walls=(1, 1, 1), tile=tile) as lat:
+            lat.set_bc_velocity(0, (0.05, 0, 0))
+            lat.set_node_types(types)
+            lat.init_uniform()
+            lat.step(80)
+            m = lat.moments()
+        if ref is None:
+            ref = m
+        else:
+            assert np.array_equal(m, ref)
+
+
+def test_reductions_match_oracle():
+    n = 32
+    with mrlbm.Lattice((n, n, n), q=19, tau=0.55) as lat:
+        lat.init_taylor_green(0.05)
+        lat.step(50)
+        gm, gk = lat.reduce()
+        mom = lat.moments()
+    orc = O.Oracle(n, n, n, q=19, tau=0.55)
+    orc.set_moments(mom)
+    om_, ok = orc.reduce()
+    say("mass", gm, om_, "ke", gk, ok)
+    assert abs(gm - om_) <= 1e-15 + 1e-12 * abs(om_)
+    assert abs(gk - ok) <= 1e-12 * ok
+
+
+def test_checkpoint_resume_bitwise(tmp_path):
+    n = 32
+    types = geometry.cavity(n)
+
+    def make():
+        lat = mrlbm.Lattice((n, n, n), q=19, tau=0.6, walls=(1, 1, 1))
+        lat.set_bc_velocity(0, (0.05, 0, 0))
+        lat.set_node_types(types)
+        lat.init_uniform()
+        return lat
+
+    a = make()
+    a.step(100)
+    straight = a.moments()
+    a.close()
+    b = make()
+    b.step(40)
+    path = tmp_path / "ck.bin"
+    b.checkpoint_save(path)
+    b.close()
+    c = make()
+    c.checkpoint_load(path)
+    assert c.info()["step"] == 40
+    c.step(60)
+    assert np.array_equal(c.moments(), straight)
+    c.close()
+
+
+def test_write_read_moments_round_trip():
+    n = 16
+    rng = np.random.default_rng(0)
+    m = rng.uniform(-0.01, 0.01, (10, n ** 3)).astype(np.float32).astype(np.float64)
+    with mrlbm.Lattice((n, n, n), q=19, tau=0.7) as lat:
+        lat.write_moments(m)
+        assert np.array_equal(lat.moments(), m)
+        # one step from these moments equals the oracle's step from the same moments
+        lat.step(1)
+        g = lat.moments()
+    orc = O.Oracle(n, n, n, q=19, tau=0.7)
+    orc.set_moments(m)
+    orc.step(1)
+    e = rel_l2(g, orc.moments())
+    assert max(e.values()) <= TOL
+
+
+def test_vtk_written(tmp_path):
+    with mrlbm.Lattice((16, 16, 16), q=19, tau=0.6) as lat:
+        lat.init_taylor_green(0.05)
+        lat.save_vtk(tmp_path / "f.vtk")
+    data = (tmp_path / "f.vtk").read_bytes()
+    assert data.startswith(b"# vtk DataFile Version 3.0")
+    assert len(data) > 16 ** 3 * 16

@@ -1,0 +1,129 @@
+"""Host-side logic of the library and the C ABI surface (CPU only: no
+compute call reaches a device)."""
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import mrlbm
+from mrlbm import _lib, geometry
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "bc_*.npz")))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names |= set(re.findall(r"\b(mrlbm_[a-z0-9_]+)\s*\(", src))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    # and the Python binding types every one of them
+    assert sorted(set(syms) - set(_lib.exported_symbols())) == []
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+def test_bc_classes_match_reference_fragments(path):
+    """The kernels' boundary tables (known mask + inverse system built on the
+    host) reproduce the reference's case formulas."""
+    d = np.load(path)
+    q = int(d["q"])
+    for t, om, pop, out in zip(d["types"], d["omega"], d["pops"], d["out"]):
+        got = mrlbm.bc_solve_host(q, int(t), float(om), out[1:4], pop)
+        np.testing.assert_allclose(got, out, rtol=1e-11, atol=1e-13)
+
+
+def test_bc_class_info_rejects_bad_lattice():
+    with pytest.raises(mrlbm.MrlbmError):
+        mrlbm.bc_class_info(15, 0x33, 1.0)
+
+
+def brute_force_types(shape, walls, solid):
+    nx, ny, nz = shape
+    t = np.zeros(shape, np.uint32)
+    n = (nx, ny, nz)
+    for x in range(nx):
+        for y in range(ny):
+            for z in range(nz):
+                if solid[x, y, z]:
+                    t[x, y, z] = 0xFF
+                    continue
+                bits = 0
+                p = (x, y, z)
+                for a in range(3):
+                    if walls[a] and p[a] == 0:
+                        bits |= geometry.octants_of(tuple(-1 if b == a else 0 for b in range(3)))
+                    if walls[a] and p[a] == n[a] - 1:
+                        bits |= geometry.octants_of(tuple(1 if b == a else 0 for b in range(3)))
+                for dx in (-1, 0, 1):
+                    for dy in (-1, 0, 1):
+                        for dz in (-1, 0, 1):
+                            if (dx, dy, dz) == (0, 0, 0):
+                                continue
+                            q = [x + dx, y + dy, z + dz]
+                            ok = True
+                            for a in range(3):
+                                if q[a] < 0 or q[a] >= n[a]:
+                                    if walls[a]:
+                                        ok = False
+                                    q[a] %= n[a]
+                            if ok and solid[q[0], q[1], q[2]]:
+                                bits |= geometry.octants_of((dx, dy, dz))
+                t[x, y, z] = bits
+    return t
+
+
+def test_geometry_classify_matches_brute_force():
+    rng = np.random.default_rng(3)
+    shape = (7, 6, 5)
+    solid = rng.random(shape) < 0.15
+    for walls in ((0, 0, 0), (0, 1, 1), (1, 1, 1)):
+        np.testing.assert_array_equal(geometry.classify(shape, walls, solid), brute_force_types(shape, walls, solid))
+
+
+def test_octant_rule_of_reference_bc_id():
+    # bc_id (src/lbmInitialization.cu:1080-1227): +x neighbour -> bits 1,3,5,7;
+    # (+1,+1,0) -> 3,7; (+1,+1,+1) -> 7; (-1,-1,-1) -> 0
+    assert geometry.octants_of((1, 0, 0)) == 0b10101010
+    assert geometry.octants_of((1, 1, 0)) == 0b10001000
+    assert geometry.octants_of((1, 1, 1)) == 0b10000000
+    assert geometry.octants_of((-1, -1, -1)) == 0b00000001
+    # the named faces of include/mrlbm_nodetype.h
+    assert geometry.octants_of((0, 1, 0)) == 0xCC and geometry.octants_of((0, 0, -1)) == 0x0F
+
+
+def test_cavity_lid_owns_edges():
+    t = geometry.cavity(8, lid_axis=2)
+    lid = t[:, :, 7]
+    assert np.all(lid & 0x100)
+    assert np.all((lid & 0xFF) & 0xF0 == 0xF0)
+    # edge of the lid plane also carries the side wall's octants
+    assert t[0, 3, 7] & 0xFF == (0xF0 | 0x55)
+    assert t[3, 3, 3] == 0
+
+
+def test_create_reports_errors_without_crashing(gpu_available):
+    if gpu_available:
+        pytest.skip("host without a GPU only")
+    with pytest.raises(mrlbm.MrlbmError):
+        mrlbm.Lattice((8, 8, 8))
+
+
+def test_invalid_parameters_rejected():
+    with pytest.raises(mrlbm.MrlbmError):
+        mrlbm.Lattice((8, 8, 8), q=15)
+    with pytest.raises(mrlbm.MrlbmError):
+        mrlbm.Lattice((8, 8, 8), tau=0.5)
+    with pytest.raises(mrlbm.MrlbmError):
+        mrlbm.Lattice((12, 8, 8), tile=(8, 8, 8))
