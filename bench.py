@@ -65,7 +65,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--tile", default="8,8,8")
+    ap.add_argument("--tile", default="", help="bx,by,bz (default: the library's choice)")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "tile", "column"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -78,17 +79,21 @@ def main():
     import mrlbm
 
     n = args.n
-    tile = tuple(int(v) for v in args.tile.split(","))
+    tile = tuple(int(v) for v in args.tile.split(",")) if args.tile else None
+    kernel = {"auto": mrlbm.KERNEL_AUTO, "tile": mrlbm.KERNEL_TILE, "column": mrlbm.KERNEL_COLUMN}[args.kernel]
     nu, u0 = 0.01, 0.05
     tau = 0.5 + 3 * nu
     N = n ** 3
-    print(f"bench: D3Q19 TGV {n}^3 tile {tile} steps {args.steps} warmup {args.warmup}", file=sys.stderr, flush=True)
+    print(f"bench: D3Q19 TGV {n}^3 kernel {args.kernel} tile {tile} steps {args.steps} warmup {args.warmup}",
+          file=sys.stderr, flush=True)
 
     copy_gbs, _ = mrlbm.copy_bandwidth(1 << 30, 20)
 
-    lat = mrlbm.Lattice((n, n, n), q=19, tau=tau, tile=tile)
+    lat = mrlbm.Lattice((n, n, n), q=19, tau=tau, tile=tile, kernel=kernel)
     lat.init_taylor_green(u0)
     info = lat.info()
+    tile = (info["bx"], info["by"], info["bz"])
+    kname = {mrlbm.KERNEL_TILE: "tile", mrlbm.KERNEL_COLUMN: "column"}[info["kernel"]]
     lat.step(args.warmup)
     lat.synchronize()
     t0 = time.perf_counter()
@@ -105,7 +110,7 @@ def main():
     avg_launch_s = ev_ms * 1e-3 / args.steps
     b_lu = info["bytes_per_lu"]
     achieved = b_lu * N / avg_launch_s / 1e9
-    key = f"tgv{n}_d3q19_fp32_t{tile[0]}x{tile[1]}x{tile[2]}"
+    key = f"tgv{n}_d3q19_fp32_{kname}{tile[0]}x{tile[1]}x{tile[2]}"
     traffic = traffic_from_profiles(key)
 
     out = {
@@ -123,10 +128,10 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"D3Q19 Taylor-Green vortex {n}^3 triply periodic, FP32 moment storage, "
-                        f"u0={u0}, nu={nu} (tau={tau}), tile {tile[0]}x{tile[1]}x{tile[2]}",
+                        f"u0={u0}, nu={nu} (tau={tau}), {kname} kernel {tile[0]}x{tile[1]}x{tile[2]}",
             "lattice": [n, n, n],
             "tile": list(tile),
-            "kernel": "tile",
+            "kernel": kname,
             "bytes_per_lu": b_lu,
             "vs_baseline_ref": "A100 11390 MLUPS, D3Q19 FP32 256^3 (reference README.md:141)",
             "copy_kernel_gbs": round(copy_gbs, 1),

@@ -146,9 +146,29 @@ int mrlbm_create(const mrlbm_params* pp, mrlbm_ctx** out) {
     if (p.kernel != MRLBM_KERNEL_AUTO && p.kernel != MRLBM_KERNEL_TILE && p.kernel != MRLBM_KERNEL_COLUMN)
         return fail(MRLBM_EINVAL, "unknown kernel family %d", p.kernel);
     int bx = p.bx, by = p.by, bz = p.bz;
-    if (bx == 0 && by == 0 && bz == 0) bx = by = bz = 8;
-    if (!tile_shape_supported(p.q, bx, by, bz, p.storage))
+    int kernel = p.kernel;
+    if (bx == 0 && by == 0 && bz == 0) {
+        // default: the column kernel with the longest segment that divides
+        // the lattice, else 8x8x8 tiles
+        const int cand[3][3] = {{16, 16, 64}, {16, 16, 32}, {8, 8, 8}};
+        for (int c = 0; c < 3; c++) {
+            const bool col = cand[c][0] * cand[c][1] == 256;
+            if (kernel == MRLBM_KERNEL_TILE && col) continue;
+            if (kernel == MRLBM_KERNEL_COLUMN && !col) continue;
+            if (p.nx % cand[c][0] == 0 && p.ny % cand[c][1] == 0 && p.nz % cand[c][2] == 0) {
+                bx = cand[c][0];
+                by = cand[c][1];
+                bz = cand[c][2];
+                break;
+            }
+        }
+        if (bx == 0) return fail(MRLBM_EINVAL, "no built tile shape divides %dx%dx%d", p.nx, p.ny, p.nz);
+    }
+    if (kernel == MRLBM_KERNEL_AUTO) kernel = (bx * by == 256 && bz > 8) ? MRLBM_KERNEL_COLUMN : MRLBM_KERNEL_TILE;
+    if (kernel == MRLBM_KERNEL_TILE && !tile_shape_supported(p.q, bx, by, bz, p.storage))
         return fail(MRLBM_EINVAL, "tile shape %dx%dx%d not built for q=%d storage=%d", bx, by, bz, p.q, p.storage);
+    if (kernel == MRLBM_KERNEL_COLUMN && !column_shape_supported(p.q, bx, by, bz, p.storage))
+        return fail(MRLBM_EINVAL, "column shape %dx%dx%d not built for q=%d storage=%d", bx, by, bz, p.q, p.storage);
     if (p.nx % bx || p.ny % by || p.nz % bz)
         return fail(MRLBM_EINVAL, "lattice %dx%dx%d is not a multiple of the tile %dx%dx%d", p.nx, p.ny, p.nz, bx, by,
                     bz);
@@ -163,7 +183,7 @@ int mrlbm_create(const mrlbm_params* pp, mrlbm_ctx** out) {
     c->p.bx = c->bx = bx;
     c->p.by = c->by = by;
     c->p.bz = c->bz = bz;
-    c->p.kernel = MRLBM_KERNEL_TILE;
+    c->p.kernel = kernel;
     c->T = bx * by * bz;
     c->ntx = p.nx / bx;
     c->nty = p.ny / by;
@@ -304,7 +324,12 @@ int upload_and_ghosts(mrlbm_ctx* c, const std::vector<float>& blk) {
     HIPCHK(hipMemcpy(c->d_mom, blk.data(), blk.size() * sizeof(float), hipMemcpyHostToDevice));
     StepArgs a = c->args();
     for (int s = 0; s < 2; s++) {
-        HIPCHK(launch_init_ghosts(c->p.q, c->bx, c->by, c->bz, c->p.storage, a, c->d_ghost[s], c->ntiles, c->stream));
+        if (c->p.kernel == MRLBM_KERNEL_COLUMN)
+            HIPCHK(launch_init_ghosts_column(c->p.q, c->bx, c->by, c->bz, c->p.storage, a, c->d_ghost[s], c->ntiles,
+                                             c->stream));
+        else
+            HIPCHK(launch_init_ghosts(c->p.q, c->bx, c->by, c->bz, c->p.storage, a, c->d_ghost[s], c->ntiles,
+                                      c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->cur = 0;
@@ -377,7 +402,9 @@ static int do_steps(mrlbm_ctx* c, int nsteps) {
     for (int s = 0; s < nsteps; s++) {
         StepArgs a = c->args();
         hipError_t e =
-            launch_step_tile(c->p.q, c->bx, c->by, c->bz, c->has_bc, c->p.storage, a, c->ntiles, c->stream);
+            c->p.kernel == MRLBM_KERNEL_COLUMN
+                ? launch_step_column(c->p.q, c->bx, c->by, c->bz, c->has_bc, c->p.storage, a, c->ntiles, c->stream)
+                : launch_step_tile(c->p.q, c->bx, c->by, c->bz, c->has_bc, c->p.storage, a, c->ntiles, c->stream);
         if (e != hipSuccess) return fail(MRLBM_EHIP, "step launch failed: %s", hipGetErrorString(e));
         c->cur = 1 - c->cur;
         c->step++;
@@ -487,7 +514,8 @@ int mrlbm_bc_class_info(int q, uint32_t node_type, double omega, const double u[
 
 // ---------------------------------------------------------------------------
 // checkpoint: header, then moments (block layout, storage type), then the
-// six faces of the ghost set the next step reads.
+// six faces of the ghost set the next step reads. Ghost values are the
+// populations as the kernels carry them, f_i - w_i RHO_0.
 
 struct CkHeader {
     char magic[8];

@@ -30,6 +30,7 @@ void hermite(int i, double h[6]) {
     h[5] = CZ[i] * CZ[i] - t;
 }
 const double COEF[6] = {1, 2, 2, 1, 2, 1};
+constexpr int BC_REC = 72;  // floats per class record (lattice.h BC_STRIDE)
 }  // namespace
 
 uint32_t known_mask(int q, uint32_t octants) {
@@ -118,7 +119,7 @@ std::string build_class_table(int q, double omega, const double bc_vel[4][3], co
     out = BcClassTable();
     out.class_of.assign(n, 0);
     out.keys.push_back(0);
-    out.table.assign(56, 0.0f);  // class 0 unused (bulk)
+    out.table.assign(BC_REC, 0.0f);  // class 0 unused (bulk)
     if (!types) return "";
     std::map<uint32_t, int> ids;
     for (size_t j = 0; j < n; j++) {
@@ -154,13 +155,34 @@ std::string build_class_table(int q, double omega, const double bc_vel[4][3], co
             id = (int)ids.size() + 1;
             ids[key] = id;
             out.keys.push_back(key);
-            float rec[56] = {};
+            float rec[BC_REC] = {};
             std::memcpy(&rec[0], &known, 4);
             rec[1] = (float)u[0];
             rec[2] = (float)u[1];
             rec[3] = (float)u[2];
             for (int k = 0; k < 49; k++) rec[4 + k] = (float)ainv[k];
-            out.table.insert(out.table.end(), rec, rec + 56);
+            // rest-state split of the right-hand side (populations travel
+            // as f_i - w_i RHO_0): W_K, c = sum_K w_i H_i / W_K, and the rest
+            // solution y_rest = A^-1 (1, c), d0 = W_K y_rest,0 - 1
+            double wk = 0, wh[6] = {0, 0, 0, 0, 0, 0};
+            for (int i = 0; i < q; i++) {
+                if (!((known >> i) & 1u)) continue;
+                double h[6];
+                hermite(i, h);
+                wk += weight(q, i);
+                for (int j = 0; j < 6; j++) wh[j] += weight(q, i) * h[j];
+            }
+            double brest[7] = {1.0, wh[0] / wk, wh[1] / wk, wh[2] / wk, wh[3] / wk, wh[4] / wk, wh[5] / wk};
+            double yrest[7];
+            for (int k = 0; k < 7; k++) {
+                yrest[k] = 0;
+                for (int j = 0; j < 7; j++) yrest[k] += ainv[7 * k + j] * brest[j];
+            }
+            rec[53] = (float)wk;
+            for (int j = 0; j < 6; j++) rec[54 + j] = (float)brest[1 + j];
+            rec[60] = (float)(wk * yrest[0] - 1.0);
+            for (int k = 0; k < 7; k++) rec[61 + k] = (float)yrest[k];
+            out.table.insert(out.table.end(), rec, rec + BC_REC);
         } else {
             id = it->second;
         }

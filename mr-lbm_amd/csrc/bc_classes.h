@@ -22,7 +22,7 @@ bool build_bc_inverse(int q, uint32_t known, double omega, const double u[3], do
 
 struct BcClassTable {
     std::vector<uint32_t> keys;      // node-type key (octants | kind | velocity index) per class, index 1..
-    std::vector<float> table;        // BC_STRIDE floats per class (class 0 unused)
+    std::vector<float> table;        // 72 floats per class (lattice.h BC_STRIDE; class 0 unused)
     std::vector<uint8_t> class_of;   // per node in the caller's x-fastest order
     bool has_bc = false, has_solid = false;
 };

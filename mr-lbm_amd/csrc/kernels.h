@@ -60,6 +60,14 @@ hipError_t launch_step_tile(int q, int bx, int by, int bz, bool has_bc, int stor
                             int ntiles, hipStream_t s);
 bool tile_shape_supported(int q, int bx, int by, int bz, int storage);
 
+// Column kernel (step_column.hip): one (x,y) column per thread over a
+// z-segment of bz planes; bx*by = 256.
+hipError_t launch_step_column(int q, int bx, int by, int bz, bool has_bc, int storage, const StepArgs& a,
+                              int ntiles, hipStream_t s);
+bool column_shape_supported(int q, int bx, int by, int bz, int storage);
+hipError_t launch_init_ghosts_column(int q, int bx, int by, int bz, int storage, const StepArgs& a,
+                                     float* const g[6], int ntiles, hipStream_t s);
+
 // Rebuild all six ghost faces of buffer `g` from the stored moments,
 // unconditionally on every tile face (src/lbmInitialization.cu:220-335).
 hipError_t launch_init_ghosts(int q, int bx, int by, int bz, int storage, const StepArgs& a, float* const g[6],

@@ -11,6 +11,12 @@
 //   rho - RHO_0, U = 3u, Mxx = 4.5 m_xx, Mxy = 9 m_xy, Mxz = 9 m_xz,
 //   Myy = 4.5 m_yy, Myz = 9 m_yz, Mzz = 4.5 m_zz,
 // with m_ab = sum_i f_i c_ia c_ib / rho - cs2 delta_ab.
+//
+// Populations travel (LDS, ghost planes) as deviations from the rest state,
+// g_i = f_i - w_i RHO_0, and density as drho = rho - RHO_0. Every sum the
+// reference forms from O(1) numbers (rho = sum f, m_aa = sum f c_a^2/rho -
+// 1/3) is then formed from small numbers, which removes its FP32
+// cancellation; the arithmetic is otherwise the reference's.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -70,15 +76,16 @@ __host__ __device__ constexpr int face_pop(int face, int slot) {
 // ---------------------------------------------------------------------------
 // FP32 node state in the scaled variables of the reference kernel.
 struct Node {
+    float drho;             // rho - RHO_0
     float rho;              // full density
     float ux, uy, uz;       // 3u
     float mxx, mxy, mxz, myy, myz, mzz;  // 4.5 m_aa / 9 m_ab
 };
 
 template <int Q, int I>
-__device__ __forceinline__ float recon_one(float pics2, const Node& n, float rw) {
+__device__ __forceinline__ float recon_one(float e0, const Node& n, float w) {
     constexpr int cx = cxv(I), cy = cyv(I), cz = czv(I);
-    float a = pics2;
+    float a = e0;
     if constexpr (cx > 0) a = a + n.ux;
     if constexpr (cx < 0) a = a - n.ux;
     if constexpr (cy > 0) a = a + n.uy;
@@ -94,32 +101,35 @@ __device__ __forceinline__ float recon_one(float pics2, const Node& n, float rw)
     if constexpr (cx * cz < 0) a = a - n.mxz;
     if constexpr (cy * cz > 0) a = a + n.myz;
     if constexpr (cy * cz < 0) a = a - n.myz;
-    return rw * a;
+    // f_i - w_i RHO_0 = w_i (drho + rho e_i), e_i = bracket - 1
+    return w * (n.drho + n.rho * a);
 }
 
 // Second-order Hermite reconstruction (src/colrec/MR_LBM/reconstruction.inc:1-47):
-// f_i = rho w_i [1 - cs2 (Mxx+Myy+Mzz) + c_i.U + sum_a c_ia^2 M_aa + sum_{a<b} c_ia c_ib M_ab].
+// f_i = rho w_i [1 - cs2 (Mxx+Myy+Mzz) + c_i.U + sum_a c_ia^2 M_aa + sum_{a<b} c_ia c_ib M_ab],
+// returned as g_i = f_i - w_i RHO_0.
 template <int Q, size_t... Is>
-__device__ __forceinline__ void reconstruct_impl(const Node& n, float* pop, std::index_sequence<Is...>) {
-    const float pics2 = 1.0f - (1.0f / 3.0f) * ((n.mxx + n.myy) + n.mzz);
-    const float rw[4] = {n.rho * (float)VS<Q>::wd(0), n.rho * (float)VS<Q>::wd(1), n.rho * (float)VS<Q>::wd(7),
-                         Q == 27 ? n.rho * (float)VS<Q>::wd(Q - 1) : 0.0f};
-    ((pop[Is] = recon_one<Q, (int)Is>(pics2, n, rw[wclass((int)Is)])), ...);
+__device__ __forceinline__ void reconstruct_impl(const Node& n, float* g, std::index_sequence<Is...>) {
+    const float e0 = -(1.0f / 3.0f) * ((n.mxx + n.myy) + n.mzz);
+    const float w[4] = {(float)VS<Q>::wd(0), (float)VS<Q>::wd(1), (float)VS<Q>::wd(7),
+                        Q == 27 ? (float)VS<Q>::wd(Q - 1) : 0.0f};
+    ((g[Is] = recon_one<Q, (int)Is>(e0, n, w[wclass((int)Is)])), ...);
 }
 template <int Q>
-__device__ __forceinline__ void reconstruct(const Node& n, float* pop) {
-    reconstruct_impl<Q>(n, pop, std::make_index_sequence<Q>{});
+__device__ __forceinline__ void reconstruct(const Node& n, float* g) {
+    reconstruct_impl<Q>(n, g, std::make_index_sequence<Q>{});
 }
 
 // Moments of the streamed populations at a bulk node
 // (src/mlbm.cu:464-509): rho, u = sum f c / rho, m_ab = sum f c_a c_b / rho - cs2 d_ab,
-// then scaled by 3, 4.5, 9.
+// then scaled by 3, 4.5, 9. From deviations: drho = sum g, j = sum g c,
+// m_aa = (sum g c_a^2 - drho/3) / rho, m_ab = sum g c_a c_b / rho.
 template <int Q>
-__device__ __forceinline__ void bulk_moments(const float* pop, Node& n) {
+__device__ __forceinline__ void bulk_moments(const float* g, float rho0, Node& n) {
     float r = 0.f, jx = 0.f, jy = 0.f, jz = 0.f, sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
 #pragma unroll
     for (int i = 0; i < Q; i++) {
-        const float f = pop[i];
+        const float f = g[i];
         r = r + f;
         if (cxv(i) > 0) jx = jx + f;
         if (cxv(i) < 0) jx = jx - f;
@@ -137,42 +147,51 @@ __device__ __forceinline__ void bulk_moments(const float* pop, Node& n) {
         if (cyv(i) * czv(i) > 0) syz = syz + f;
         if (cyv(i) * czv(i) < 0) syz = syz - f;
     }
-    const float inv = 1.0f / r;
-    const float cs2 = 1.0f / 3.0f;
-    n.rho = r;
+    n.drho = r;
+    n.rho = rho0 + r;
+    const float inv = 1.0f / n.rho;
+    const float r3 = r * (1.0f / 3.0f);
     n.ux = 3.0f * (jx * inv);
     n.uy = 3.0f * (jy * inv);
     n.uz = 3.0f * (jz * inv);
-    n.mxx = 4.5f * (sxx * inv - cs2);
-    n.myy = 4.5f * (syy * inv - cs2);
-    n.mzz = 4.5f * (szz * inv - cs2);
+    n.mxx = 4.5f * ((sxx - r3) * inv);
+    n.myy = 4.5f * ((syy - r3) * inv);
+    n.mzz = 4.5f * ((szz - r3) * inv);
     n.mxy = 9.0f * (sxy * inv);
     n.mxz = 9.0f * (sxz * inv);
     n.myz = 9.0f * (syz * inv);
 }
 
-// Boundary-condition class record (device table, 56 floats per class).
-//   [0]      bit mask of known populations (bit i set = population i known)
-//   [1..3]   wall velocity (unscaled)
-//   [4..52]  7x7 inverse of the moment-matching system, row-major
+// Boundary-condition class record (device table, BC_STRIDE floats per class).
+//   [0]       bit mask of known populations (bit i set = population i known)
+//   [1..3]    wall velocity (unscaled)
+//   [4..52]   A^-1, inverse of the 7x7 moment-matching system, row-major
+//   [53]      W_K = sum_{i in K} w_i
+//   [54..59]  c_k = sum_{i in K} w_i H_i,k / W_K   (k = xx, xy, xz, yy, yz, zz)
+//   [60]      d0  = W_K y_rest,0 - 1
+//   [61..67]  y_rest = A^-1 (1, c)  (the solution for the rest state)
 // The system (one per class, built on the host in FP64) is the one whose
 // closed-form solutions are the reference's generated bc_definition.inc
 // formulas: with P = rho m,
 //   sum_{i in K} f^R_i(rho, u_w, (1-omega) m + omega u_w u_w) = rho_I
 //   sum_{i in K} f^R_i(rho, u_w, m) H_i,ab                     = rho_I m_I,ab
-// where K is the known set, rho_I = sum_K f_i and m_I = sum_K f_i H_i / rho_I.
-constexpr int BC_STRIDE = 56;
+// where K is the known set, rho_I = sum_K f_i and m_I = sum_K f_i H_i / rho_I;
+// x = (rho, P) = rho_I A^-1 (1, m_I).
+constexpr int BC_STRIDE = 72;
 
-// Moment-based boundary condition at a node of class `c`.
-// Outputs unscaled rho, u, m exactly as a bc_definition.inc fragment does
-// (src/mlbm.cu:456-461); the caller scales them.
+// Moment-based boundary condition at a node of class `c` from the streamed
+// deviations g (f_i = w_i RHO_0 + g_i). The solve is split around the rest
+// state: (1, m_I) = (1, c) + (0, db) with db = (T - c r) / rho_I, where
+// r = sum_K g and T = sum_K g H, so every term the FP32 arithmetic forms is
+// a small deviation. Outputs unscaled rho, u, m exactly as a
+// bc_definition.inc fragment does (src/mlbm.cu:456-461); the caller scales.
 template <int Q>
-__device__ __forceinline__ void bc_solve(const float* __restrict__ c, const float* pop, Node& n) {
+__device__ __forceinline__ void bc_solve(const float* __restrict__ c, const float* g, float rho0, Node& n) {
     const uint32_t known = __float_as_uint(c[0]);
     float r = 0.f, sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
 #pragma unroll
     for (int i = 0; i < Q; i++) {
-        const float f = ((known >> i) & 1u) ? pop[i] : 0.0f;
+        const float f = ((known >> i) & 1u) ? g[i] : 0.0f;
         r = r + f;
         if (cxv(i) != 0) sxx = sxx + f;
         if (cyv(i) != 0) syy = syy + f;
@@ -184,35 +203,38 @@ __device__ __forceinline__ void bc_solve(const float* __restrict__ c, const floa
         if (cyv(i) * czv(i) > 0) syz = syz + f;
         if (cyv(i) * czv(i) < 0) syz = syz - f;
     }
-    const float inv = 1.0f / r;
-    const float cs2 = 1.0f / 3.0f;
-    float b[7];
-    b[0] = 1.0f;
-    b[1] = sxx * inv - cs2;
-    b[2] = sxy * inv;
-    b[3] = sxz * inv;
-    b[4] = syy * inv - cs2;
-    b[5] = syz * inv;
-    b[6] = szz * inv - cs2;
-    float y[7];
+    const float rI = rho0 * c[53] + r;
+    const float inv = 1.0f / rI;
+    const float r3 = r * (1.0f / 3.0f);
+    float db[6];
+    db[0] = ((sxx - r3) - c[54] * r) * inv;
+    db[1] = (sxy - c[55] * r) * inv;
+    db[2] = (sxz - c[56] * r) * inv;
+    db[3] = ((syy - r3) - c[57] * r) * inv;
+    db[4] = (syz - c[58] * r) * inv;
+    db[5] = ((szz - r3) - c[59] * r) * inv;
+    float dy[7];
 #pragma unroll
     for (int k = 0; k < 7; k++) {
         float acc = 0.0f;
 #pragma unroll
-        for (int j = 0; j < 7; j++) acc = acc + c[4 + 7 * k + j] * b[j];
-        y[k] = acc;
+        for (int j = 0; j < 6; j++) acc = acc + c[4 + 7 * k + 1 + j] * db[j];
+        dy[k] = acc;
     }
-    const float iy0 = 1.0f / y[0];
-    n.rho = r * y[0];
+    const float y0 = c[61] + dy[0];
+    const float iy0 = 1.0f / y0;
+    // rho - RHO_0 = rho_I y0 - RHO_0 = RHO_0 d0 + r y_rest,0 + rho_I dy0
+    n.drho = (rho0 * c[60] + r * c[61]) + rI * dy[0];
+    n.rho = rho0 + n.drho;
     n.ux = c[1];
     n.uy = c[2];
     n.uz = c[3];
-    n.mxx = y[1] * iy0;
-    n.mxy = y[2] * iy0;
-    n.mxz = y[3] * iy0;
-    n.myy = y[4] * iy0;
-    n.myz = y[5] * iy0;
-    n.mzz = y[6] * iy0;
+    n.mxx = (c[62] + dy[1]) * iy0;
+    n.mxy = (c[63] + dy[2]) * iy0;
+    n.mxz = (c[64] + dy[3]) * iy0;
+    n.myy = (c[65] + dy[4]) * iy0;
+    n.myz = (c[66] + dy[5]) * iy0;
+    n.mzz = (c[67] + dy[6]) * iy0;
 }
 
 // Scale unscaled u, m to the stored convention (src/mlbm.cu:500-509).

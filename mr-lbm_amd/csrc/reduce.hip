@@ -24,7 +24,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_partial(const void* mom_
         const float dr = C::ld(mom, b, rs0);
         const double ux = C::ld(mom, b + T, rs1), uy = C::ld(mom, b + 2 * T, rs1), uz = C::ld(mom, b + 3 * T, rs1);
         am += (double)dr;
-        ak += (double)(rho0 + dr) * (ux * ux + uy * uy + uz * uz) / 18.0;
+        ak += ((double)rho0 + (double)dr) * (ux * ux + uy * uy + uz * uz) / 18.0;
     }
     sm[threadIdx.x] = am;
     sk[threadIdx.x] = ak;

@@ -158,18 +158,29 @@ def bc_class_info(q, node_type, omega, u=(0.0, 0.0, 0.0)):
     return known.value, a.reshape(7, 7)
 
 
-def bc_solve_host(q, node_type, omega, u, pop):
+def bc_solve_host(q, node_type, omega, u, pop, rho0=1.0):
     """FP64 evaluation of the kernels' boundary rule for one node (host, for
-    checks): unscaled rho, u, m as a bc_definition.inc fragment leaves them."""
+    checks), in the kernels' formulation: populations as deviations
+    g = f - w rho0 and the solve split around the rest state. Returns
+    unscaled rho, u, m as a bc_definition.inc fragment leaves them."""
     known, ainv = bc_class_info(q, node_type, omega, u)
     c = np.array([[0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1],
                   [0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1],
                   [0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1]])[:, :q]
+    kcls = np.array([0 if i == 0 else (1 if i < 7 else (2 if i < 19 else 3)) for i in range(q)])
+    w = (np.array([1 / 3, 1 / 18, 1 / 36]) if q == 19 else np.array([8 / 27, 2 / 27, 1 / 54, 1 / 216]))[kcls]
     k = np.array([(known >> i) & 1 for i in range(q)], bool)
-    f = np.where(k, pop, 0.0)
-    r = f.sum()
     t = 1.0 / 3.0
-    b = np.array([1.0, (f * c[0] * c[0]).sum() / r - t, (f * c[0] * c[1]).sum() / r, (f * c[0] * c[2]).sum() / r,
-                  (f * c[1] * c[1]).sum() / r - t, (f * c[1] * c[2]).sum() / r, (f * c[2] * c[2]).sum() / r - t])
-    y = ainv @ b
-    return np.concatenate([[r * y[0]], np.asarray(u, float), y[1:] / y[0]])
+    H = np.stack([c[0] * c[0] - t, c[0] * c[1], c[0] * c[2], c[1] * c[1] - t, c[1] * c[2], c[2] * c[2] - t])
+    g = np.where(k, np.asarray(pop, float) - w * rho0, 0.0)
+    wk = w[k].sum()
+    cvec = (H[:, k] * w[k]).sum(axis=1) / wk
+    yrest = ainv @ np.concatenate([[1.0], cvec])
+    r = g.sum()
+    T = H @ g
+    rI = rho0 * wk + r
+    db = (T - cvec * r) / rI
+    dy = ainv[:, 1:] @ db
+    y = yrest + dy
+    drho = rho0 * (wk * yrest[0] - 1.0) + r * yrest[0] + rI * dy[0]
+    return np.concatenate([[rho0 + drho], np.asarray(u, float), y[1:] / y[0]])
