@@ -100,6 +100,7 @@ struct mrlbm_ctx {
         a.R.fx = (float)p.fx;
         a.R.fy = (float)p.fy;
         a.R.fz = (float)p.fz;
+        a.R.has_force = (p.fx != 0.0 || p.fy != 0.0 || p.fz != 0.0) ? 1 : 0;
         for (int m = 0; m < 10; m++) a.mscale[m] = a.mrscale[m] = 1.0f;
         return a;
     }

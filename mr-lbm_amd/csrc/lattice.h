@@ -78,88 +78,130 @@ __host__ __device__ constexpr int face_pop(int face, int slot) {
 struct Node {
     float drho;             // rho - RHO_0
     float rho;              // full density
+    float inv;              // 1 / rho (set by the moment evaluation)
     float ux, uy, uz;       // 3u
     float mxx, mxy, mxz, myy, myz, mzz;  // 4.5 m_aa / 9 m_ab
 };
 
-template <int Q, int I>
-__device__ __forceinline__ float recon_one(float e0, const Node& n, float w) {
-    constexpr int cx = cxv(I), cy = cyv(I), cz = czv(I);
-    float a = e0;
-    if constexpr (cx > 0) a = a + n.ux;
-    if constexpr (cx < 0) a = a - n.ux;
-    if constexpr (cy > 0) a = a + n.uy;
-    if constexpr (cy < 0) a = a - n.uy;
-    if constexpr (cz > 0) a = a + n.uz;
-    if constexpr (cz < 0) a = a - n.uz;
-    if constexpr (cx != 0) a = a + n.mxx;
-    if constexpr (cy != 0) a = a + n.myy;
-    if constexpr (cz != 0) a = a + n.mzz;
-    if constexpr (cx * cy > 0) a = a + n.mxy;
-    if constexpr (cx * cy < 0) a = a - n.mxy;
-    if constexpr (cx * cz > 0) a = a + n.mxz;
-    if constexpr (cx * cz < 0) a = a - n.mxz;
-    if constexpr (cy * cz > 0) a = a + n.myz;
-    if constexpr (cy * cz < 0) a = a - n.myz;
-    // f_i - w_i RHO_0 = w_i (drho + rho e_i), e_i = bracket - 1
-    return w * (n.drho + n.rho * a);
-}
-
 // Second-order Hermite reconstruction (src/colrec/MR_LBM/reconstruction.inc:1-47):
 // f_i = rho w_i [1 - cs2 (Mxx+Myy+Mzz) + c_i.U + sum_a c_ia^2 M_aa + sum_{a<b} c_ia c_ib M_ab],
-// returned as g_i = f_i - w_i RHO_0.
-template <int Q, size_t... Is>
-__device__ __forceinline__ void reconstruct_impl(const Node& n, float* g, std::index_sequence<Is...>) {
-    const float e0 = -(1.0f / 3.0f) * ((n.mxx + n.myy) + n.mzz);
-    const float w[4] = {(float)VS<Q>::wd(0), (float)VS<Q>::wd(1), (float)VS<Q>::wd(7),
-                        Q == 27 ? (float)VS<Q>::wd(Q - 1) : 0.0f};
-    ((g[Is] = recon_one<Q, (int)Is>(e0, n, w[wclass((int)Is)])), ...);
-}
+// returned as g_i = f_i - w_i RHO_0 = w_i drho + w_i rho e_i (e_i = bracket - 1).
+// Opposite populations share the even part of the bracket, so each pair
+// costs two adds and two FMAs once the rho-scaled moments are formed.
 template <int Q>
 __device__ __forceinline__ void reconstruct(const Node& n, float* g) {
-    reconstruct_impl<Q>(n, g, std::make_index_sequence<Q>{});
+    constexpr float w0 = (float)VS<Q>::wd(0), w1 = (float)VS<Q>::wd(1), w2 = (float)VS<Q>::wd(7);
+    const float R = n.rho;
+    const float re0 = R * (-(1.0f / 3.0f) * ((n.mxx + n.myy) + n.mzz));
+    const float rux = R * n.ux, ruy = R * n.uy, ruz = R * n.uz;
+    const float rxx = R * n.mxx, ryy = R * n.myy, rzz = R * n.mzz;
+    const float rxy = R * n.mxy, rxz = R * n.mxz, ryz = R * n.myz;
+    const float d0 = w0 * n.drho, d1 = w1 * n.drho, d2 = w2 * n.drho;
+    g[0] = __builtin_fmaf(w0, re0, d0);
+    const float ex = re0 + rxx, ey = re0 + ryy, ez = re0 + rzz;
+    g[1] = __builtin_fmaf(w1, ex + rux, d1);
+    g[2] = __builtin_fmaf(w1, ex - rux, d1);
+    g[3] = __builtin_fmaf(w1, ey + ruy, d1);
+    g[4] = __builtin_fmaf(w1, ey - ruy, d1);
+    g[5] = __builtin_fmaf(w1, ez + ruz, d1);
+    g[6] = __builtin_fmaf(w1, ez - ruz, d1);
+    const float exy = ex + ryy, exz = ex + rzz, eyz = ey + rzz;
+    {
+        const float p = exy + rxy, m = exy - rxy, s = rux + ruy, d = rux - ruy;
+        g[7] = __builtin_fmaf(w2, p + s, d2);
+        g[8] = __builtin_fmaf(w2, p - s, d2);
+        g[13] = __builtin_fmaf(w2, m + d, d2);
+        g[14] = __builtin_fmaf(w2, m - d, d2);
+    }
+    {
+        const float p = exz + rxz, m = exz - rxz, s = rux + ruz, d = rux - ruz;
+        g[9] = __builtin_fmaf(w2, p + s, d2);
+        g[10] = __builtin_fmaf(w2, p - s, d2);
+        g[15] = __builtin_fmaf(w2, m + d, d2);
+        g[16] = __builtin_fmaf(w2, m - d, d2);
+    }
+    {
+        const float p = eyz + ryz, m = eyz - ryz, s = ruy + ruz, d = ruy - ruz;
+        g[11] = __builtin_fmaf(w2, p + s, d2);
+        g[12] = __builtin_fmaf(w2, p - s, d2);
+        g[17] = __builtin_fmaf(w2, m + d, d2);
+        g[18] = __builtin_fmaf(w2, m - d, d2);
+    }
+    if constexpr (Q == 27) {
+        constexpr float w3 = (float)VS<Q>::wd(26);
+        const float d3 = w3 * n.drho;
+        const float exyz = exy + rzz;
+        const float a = exyz + rxy, b = exyz - rxy;
+        const float e1 = a + (rxz + ryz), e2 = a - (rxz + ryz), e3 = b + (rxz - ryz), e4 = b - (rxz - ryz);
+        const float s = rux + ruy, t = rux - ruy;
+        const float o1 = s + ruz, o2 = s - ruz, o3 = t + ruz, o4 = t - ruz;
+        g[19] = __builtin_fmaf(w3, e1 + o1, d3);  // (+,+,+)
+        g[20] = __builtin_fmaf(w3, e1 - o1, d3);  // (-,-,-)
+        g[21] = __builtin_fmaf(w3, e2 + o2, d3);  // (+,+,-)
+        g[22] = __builtin_fmaf(w3, e2 - o2, d3);  // (-,-,+)
+        g[23] = __builtin_fmaf(w3, e3 + o3, d3);  // (+,-,+)
+        g[24] = __builtin_fmaf(w3, e3 - o3, d3);  // (-,+,-)
+        g[26] = __builtin_fmaf(w3, e4 + o4, d3);  // (+,-,-)
+        g[25] = __builtin_fmaf(w3, e4 - o4, d3);  // (-,+,+)
+    }
 }
 
 // Moments of the streamed populations at a bulk node
 // (src/mlbm.cu:464-509): rho, u = sum f c / rho, m_ab = sum f c_a c_b / rho - cs2 d_ab,
 // then scaled by 3, 4.5, 9. From deviations: drho = sum g, j = sum g c,
-// m_aa = (sum g c_a^2 - drho/3) / rho, m_ab = sum g c_a c_b / rho.
+// m_aa = (sum g c_a^2 - drho/3) / rho, m_ab = sum g c_a c_b / rho. Sums are
+// formed from the pair sums p = g_i + g_i' and differences d = g_i - g_i'.
 template <int Q>
 __device__ __forceinline__ void bulk_moments(const float* g, float rho0, Node& n) {
-    float r = 0.f, jx = 0.f, jy = 0.f, jz = 0.f, sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
-#pragma unroll
-    for (int i = 0; i < Q; i++) {
-        const float f = g[i];
-        r = r + f;
-        if (cxv(i) > 0) jx = jx + f;
-        if (cxv(i) < 0) jx = jx - f;
-        if (cyv(i) > 0) jy = jy + f;
-        if (cyv(i) < 0) jy = jy - f;
-        if (czv(i) > 0) jz = jz + f;
-        if (czv(i) < 0) jz = jz - f;
-        if (cxv(i) != 0) sxx = sxx + f;
-        if (cyv(i) != 0) syy = syy + f;
-        if (czv(i) != 0) szz = szz + f;
-        if (cxv(i) * cyv(i) > 0) sxy = sxy + f;
-        if (cxv(i) * cyv(i) < 0) sxy = sxy - f;
-        if (cxv(i) * czv(i) > 0) sxz = sxz + f;
-        if (cxv(i) * czv(i) < 0) sxz = sxz - f;
-        if (cyv(i) * czv(i) > 0) syz = syz + f;
-        if (cyv(i) * czv(i) < 0) syz = syz - f;
+    const float p12 = g[1] + g[2], d12 = g[1] - g[2];
+    const float p34 = g[3] + g[4], d34 = g[3] - g[4];
+    const float p56 = g[5] + g[6], d56 = g[5] - g[6];
+    const float p78 = g[7] + g[8], d78 = g[7] - g[8];
+    const float p910 = g[9] + g[10], d910 = g[9] - g[10];
+    const float p1112 = g[11] + g[12], d1112 = g[11] - g[12];
+    const float p1314 = g[13] + g[14], d1314 = g[13] - g[14];
+    const float p1516 = g[15] + g[16], d1516 = g[15] - g[16];
+    const float p1718 = g[17] + g[18], d1718 = g[17] - g[18];
+    float sxx = ((p12 + p78) + (p910 + p1314)) + p1516;
+    float syy = ((p34 + p78) + (p1112 + p1314)) + p1718;
+    float szz = ((p56 + p910) + (p1112 + p1516)) + p1718;
+    float sxy = p78 - p1314, sxz = p910 - p1516, syz = p1112 - p1718;
+    float jx = ((d12 + d78) + (d910 + d1314)) + d1516;
+    float jy = ((d34 + d78) + (d1112 - d1314)) + d1718;
+    float jz = ((d56 + d910) + (d1112 - d1516)) - d1718;
+    float r = (((g[0] + p12) + (p34 + p56)) + ((p78 + p910) + (p1112 + p1314))) + (p1516 + p1718);
+    if constexpr (Q == 27) {
+        const float pa = g[19] + g[20], da = g[19] - g[20];  // (+,+,+)
+        const float pb = g[21] + g[22], db = g[21] - g[22];  // (+,+,-)
+        const float pc = g[23] + g[24], dc = g[23] - g[24];  // (+,-,+)
+        const float pd = g[26] + g[25], dd = g[26] - g[25];  // (+,-,-)
+        const float pall = (pa + pb) + (pc + pd);
+        r = r + pall;
+        sxx = sxx + pall;
+        syy = syy + pall;
+        szz = szz + pall;
+        sxy = sxy + ((pa + pb) - (pc + pd));
+        sxz = sxz + ((pa - pb) + (pc - pd));
+        syz = syz + ((pa - pb) - (pc - pd));
+        jx = jx + ((da + db) + (dc + dd));
+        jy = jy + ((da + db) - (dc + dd));
+        jz = jz + ((da - db) + (dc - dd));
     }
     n.drho = r;
     n.rho = rho0 + r;
-    const float inv = 1.0f / n.rho;
+    const float inv = __builtin_amdgcn_rcpf(n.rho);
+    n.inv = inv;
+    const float u3 = 3.0f * inv, m45 = 4.5f * inv, m9 = 9.0f * inv;
     const float r3 = r * (1.0f / 3.0f);
-    n.ux = 3.0f * (jx * inv);
-    n.uy = 3.0f * (jy * inv);
-    n.uz = 3.0f * (jz * inv);
-    n.mxx = 4.5f * ((sxx - r3) * inv);
-    n.myy = 4.5f * ((syy - r3) * inv);
-    n.mzz = 4.5f * ((szz - r3) * inv);
-    n.mxy = 9.0f * (sxy * inv);
-    n.mxz = 9.0f * (sxz * inv);
-    n.myz = 9.0f * (syz * inv);
+    n.ux = jx * u3;
+    n.uy = jy * u3;
+    n.uz = jz * u3;
+    n.mxx = (sxx - r3) * m45;
+    n.myy = (syy - r3) * m45;
+    n.mzz = (szz - r3) * m45;
+    n.mxy = sxy * m9;
+    n.mxz = sxz * m9;
+    n.myz = syz * m9;
 }
 
 // Boundary-condition class record (device table, BC_STRIDE floats per class).
@@ -226,6 +268,7 @@ __device__ __forceinline__ void bc_solve(const float* __restrict__ c, const floa
     // rho - RHO_0 = rho_I y0 - RHO_0 = RHO_0 d0 + r y_rest,0 + rho_I dy0
     n.drho = (rho0 * c[60] + r * c[61]) + rI * dy[0];
     n.rho = rho0 + n.drho;
+    n.inv = __builtin_amdgcn_rcpf(n.rho);
     n.ux = c[1];
     n.uy = c[2];
     n.uz = c[3];
@@ -253,23 +296,46 @@ __device__ __forceinline__ void scale_moments(Node& n) {
 struct Relax {
     float omega, t_omega, tt_omega, omega_d2, tt_omega_t3;
     float fx, fy, fz;
+    int has_force;
 };
 
 // Second-order moment collision with force terms
-// (src/colrec/MR_LBM/collision.inc:2-14), in scaled variables.
+// (src/colrec/MR_LBM/collision.inc:2-14), in scaled variables:
+//   U += 3 F / rho
+//   M_aa = (1-w) M_aa + (w/2) U_a^2 + (3/(2 rho)) (1-w/2) 2 F_a U_a
+//   M_ab = (1-w) M_ab +  w U_a U_b  + (3/rho) (1-w/2) (F_a U_b + F_b U_a)
+// The force terms are skipped (uniform branch) when F = 0.
 __device__ __forceinline__ void collide(Node& n, const Relax& R) {
-    const float invRho = 1.0f / n.rho;
-    n.ux = n.ux + (3.0f * invRho) * R.fx;
-    n.uy = n.uy + (3.0f * invRho) * R.fy;
-    n.uz = n.uz + (3.0f * invRho) * R.fz;
-    const float i15 = (3.0f * invRho) / 2.0f;
-    const float i3 = R.tt_omega_t3 * invRho;
-    n.mxx = (R.t_omega * n.mxx + R.omega_d2 * n.ux * n.ux) + i15 * R.tt_omega * (R.fx * n.ux + R.fx * n.ux);
-    n.myy = (R.t_omega * n.myy + R.omega_d2 * n.uy * n.uy) + i15 * R.tt_omega * (R.fy * n.uy + R.fy * n.uy);
-    n.mzz = (R.t_omega * n.mzz + R.omega_d2 * n.uz * n.uz) + i15 * R.tt_omega * (R.fz * n.uz + R.fz * n.uz);
-    n.mxy = (R.t_omega * n.mxy + R.omega * n.ux * n.uy) + i3 * (R.fx * n.uy + R.fy * n.ux);
-    n.mxz = (R.t_omega * n.mxz + R.omega * n.ux * n.uz) + i3 * (R.fx * n.uz + R.fz * n.ux);
-    n.myz = (R.t_omega * n.myz + R.omega * n.uy * n.uz) + i3 * (R.fy * n.uz + R.fz * n.uy);
+    if (R.has_force) {
+        const float i3 = 3.0f * n.inv;
+        n.ux = __builtin_fmaf(i3, R.fx, n.ux);
+        n.uy = __builtin_fmaf(i3, R.fy, n.uy);
+        n.uz = __builtin_fmaf(i3, R.fz, n.uz);
+    }
+    const float hx = R.omega_d2 * n.ux, hy = R.omega_d2 * n.uy, hz = R.omega_d2 * n.uz;
+    const float ox = R.omega * n.ux, oy = R.omega * n.uy;
+    float mxx = __builtin_fmaf(R.t_omega, n.mxx, hx * n.ux);
+    float myy = __builtin_fmaf(R.t_omega, n.myy, hy * n.uy);
+    float mzz = __builtin_fmaf(R.t_omega, n.mzz, hz * n.uz);
+    float mxy = __builtin_fmaf(R.t_omega, n.mxy, ox * n.uy);
+    float mxz = __builtin_fmaf(R.t_omega, n.mxz, ox * n.uz);
+    float myz = __builtin_fmaf(R.t_omega, n.myz, oy * n.uz);
+    if (R.has_force) {
+        const float i15 = (1.5f * n.inv) * R.tt_omega;
+        const float i3 = R.tt_omega_t3 * n.inv;
+        mxx = __builtin_fmaf(i15, 2.0f * (R.fx * n.ux), mxx);
+        myy = __builtin_fmaf(i15, 2.0f * (R.fy * n.uy), myy);
+        mzz = __builtin_fmaf(i15, 2.0f * (R.fz * n.uz), mzz);
+        mxy = __builtin_fmaf(i3, R.fx * n.uy + R.fy * n.ux, mxy);
+        mxz = __builtin_fmaf(i3, R.fx * n.uz + R.fz * n.ux, mxz);
+        myz = __builtin_fmaf(i3, R.fy * n.uz + R.fz * n.uy, myz);
+    }
+    n.mxx = mxx;
+    n.myy = myy;
+    n.mzz = mzz;
+    n.mxy = mxy;
+    n.mxz = mxz;
+    n.myz = myz;
 }
 
 }  // namespace mrlbm

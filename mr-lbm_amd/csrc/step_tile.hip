@@ -6,88 +6,175 @@
 //      contiguous 256-byte runs per moment plane)
 //   2. reconstruct the post-collision populations of the previous step
 //      (as deviations g_i = f_i - w_i RHO_0, see lattice.h)
-//   3. write populations 1..Q-1 to LDS planes, barrier, pull from the
-//      in-tile upwind neighbour (periodic inside the tile)
-//   4. tile-face nodes overwrite the populations that crossed the tile
-//      face with the values from the neighbour tile's ghost planes
-//      (src/includeFiles/popLoad.inc:2-86; X, then Y, then Z branch, the
-//      later branch winning as in the reference)
+//   3. write populations 1..Q-1 to LDS arrays with a one-cell halo on the
+//      upwind side; whole waves copy the neighbours' ghost planes into the
+//      halo (halo.h; the X, Y, Z winner rule of
+//      src/includeFiles/popLoad.inc:2-86); barrier; every node pulls its
+//      populations at compile-time offsets from its own base address
 //   5. boundary class -> moment-based BC, else bulk moments; scale; collide
-//   6. store moments in place, reconstruct, write outgoing ghost planes
+//   6. store moments in place, reconstruct, stage the outgoing populations
+//      in LDS and let whole waves write the ghost planes in contiguous runs
 //      (src/includeFiles/popSave.inc:2-78 with the wall predicates of
 //      src/includeFiles/interface.h:6-47)
-// Solid nodes run the same code (no early exit before the barrier) and
-// simply store nothing.
+// Solid nodes run the same code (no early exit before a barrier) and store
+// nothing.
+#include "halo.h"
 #include "tile_common.h"
 
 namespace mrlbm {
 
+// Neighbour-tile indices of a workgroup (uniform): wrapped coordinates
+// along each axis for offsets -1, 0, +1.
+struct Nbr {
+    int x[3], y[3], z[3];
+    int ntx, nty;
+    __device__ __forceinline__ int tile(int dx, int dy, int dz) const {
+        return x[dx + 1] + ntx * (y[dy + 1] + nty * z[dz + 1]);
+    }
+};
+
+template <int Q, int BX, int BY, int BZ, int K>
+__device__ __forceinline__ void halo_load(const StepArgs& a, int lane, const Nbr& nb, float* s) {
+    constexpr auto CH = incoming_chunks<Q, BX, BY, BZ>();
+    constexpr Chunk c = CH[K];
+    constexpr int QF = VS<Q>::QF;
+    constexpr int Bu = c.face < 2 ? BY : BX;
+    constexpr int Bv = c.face < 4 ? BZ : BY;
+    const int otile = nb.tile(c.dx, c.dy, c.dz);
+    const float* __restrict__ src = a.gin[c.face] + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)otile);
+    if (lane < c.nu * c.nv) {
+        const int lu = lane % c.nu, lv = lane / c.nu;
+        const float v = src[(uint32_t)((c.u0 + lu) + Bu * (c.v0 + lv))];
+        s[c.lds0 + lu * c.su + lv * c.sv] = v;
+    }
+}
+
+template <int Q, int BX, int BY, int BZ, bool HAS_BC, int K>
+__device__ __forceinline__ void halo_store(const StepArgs& a, int lane, int tile, int bx, int by, int bz,
+                                           const float* s, const uint8_t* s_cls) {
+    constexpr auto CH = outgoing_chunks<Q, BX, BY, BZ>();
+    constexpr Chunk c = CH[K];
+    constexpr int QF = VS<Q>::QF;
+    constexpr int Bu = c.face < 2 ? BY : BX;
+    constexpr int Bv = c.face < 4 ? BZ : BY;
+    // wall predicate of src/includeFiles/interface.h:6-47: a tile face on a
+    // wall-axis domain boundary is not written
+    bool skip = false;
+    if constexpr (c.face == 0) skip = a.wall_x && bx == 0;
+    if constexpr (c.face == 1) skip = a.wall_x && bx == a.ntx - 1;
+    if constexpr (c.face == 2) skip = a.wall_y && by == 0;
+    if constexpr (c.face == 3) skip = a.wall_y && by == a.nty - 1;
+    if constexpr (c.face == 4) skip = a.wall_z && bz == 0;
+    if constexpr (c.face == 5) skip = a.wall_z && bz == a.ntz - 1;
+    if (skip) return;
+    float* __restrict__ dst = a.gout[c.face] + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)tile);
+    if (lane < c.nu * c.nv) {
+        const int lu = lane % c.nu, lv = lane / c.nu;
+        const int u = c.u0 + lu, v = c.v0 + lv;
+        if constexpr (HAS_BC) {
+            // node of the face cell (solid nodes write no ghost values)
+            constexpr int ax = c.face / 2;
+            const int fix = (c.face & 1) ? (ax == 0 ? BX : (ax == 1 ? BY : BZ)) - 1 : 0;
+            const int nx = ax == 0 ? fix : u;
+            const int ny = ax == 1 ? fix : (ax == 0 ? u : v);
+            const int nz = ax == 2 ? fix : v;
+            if (s_cls[nx + BX * (ny + BY * nz)] == 255u) return;
+        }
+        dst[(uint32_t)(u + Bu * v)] = s[c.lds0 + lu * c.su + lv * c.sv];
+    }
+}
+
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
 __global__ __launch_bounds__(BX* BY* BZ) void k_step_tile(StepArgs a) {
     constexpr int T = BX * BY * BZ;
-    __shared__ float s_pop[(Q - 1) * T];
+    constexpr int NW = T / 64;
+    using G = HaloGeom<BX, BY, BZ>;
+    constexpr int E = G::E;
+    constexpr int NIN = n_incoming_chunks<Q, BX, BY, BZ>();
+    constexpr int NOUT = n_outgoing_chunks<Q, BX, BY, BZ>();
+    __shared__ float s[(Q - 1) * E];
+    __shared__ uint8_t s_cls[HAS_BC ? T : 1];
 
     const int t = threadIdx.x;
     const int tx = t % BX, ty = (t / BX) % BY, tz = t / (BX * BY);
+    const int lane = t % 64;
+    const int w = __builtin_amdgcn_readfirstlane(t / 64);
     const int tile = blockIdx.x;
     const int bx = tile % a.ntx, by = (tile / a.ntx) % a.nty, bz = tile / (a.ntx * a.nty);
-    const int x = bx * BX + tx, y = by * BY + ty, z = bz * BZ + tz;
+    const int base = tx + G::DX * (ty + G::DY * tz);
+    // periodic tile neighbours, as the reference's bxm1/bxp1 (src/mlbm.cu:115-122)
+    Nbr nb;
+    nb.ntx = a.ntx;
+    nb.nty = a.nty;
+    nb.x[0] = bx == 0 ? a.ntx - 1 : bx - 1, nb.x[1] = bx, nb.x[2] = bx == a.ntx - 1 ? 0 : bx + 1;
+    nb.y[0] = by == 0 ? a.nty - 1 : by - 1, nb.y[1] = by, nb.y[2] = by == a.nty - 1 ? 0 : by + 1;
+    nb.z[0] = bz == 0 ? a.ntz - 1 : bz - 1, nb.z[1] = bz, nb.z[2] = bz == a.ntz - 1 ? 0 : bz + 1;
 
     uint32_t cls = 0;
-    if constexpr (HAS_BC) cls = a.cls[(size_t)tile * T + t];
+    if constexpr (HAS_BC) {
+        cls = a.cls[(size_t)tile * T + t];
+        s_cls[t] = (uint8_t)cls;
+    }
     const bool solid = HAS_BC && (cls == 255u);
 
     const size_t mb = (size_t)tile * 10 * T + t;
     Node n;
     load_node<Q, BX, BY, BZ, S>(a, mb, n);
 
-    float pop[Q];
-    reconstruct<Q>(n, pop);
+    // populations entering the tile: neighbour ghost planes -> LDS halo
+    static_for<NW>([&](auto WW) {
+        constexpr int W = decltype(WW)::value;
+        if (w == W) {
+            static_for<(NIN + NW - 1) / NW>([&](auto RR) {
+                constexpr int K = W + NW * decltype(RR)::value;
+                if constexpr (K < NIN) halo_load<Q, BX, BY, BZ, K>(a, lane, nb, s);
+            });
+        }
+    });
+
+    float g[Q];
+    reconstruct<Q>(n, g);
     static_for<Q - 1>([&](auto I) {
         constexpr int i = decltype(I)::value + 1;
-        s_pop[(i - 1) * T + t] = pop[i];
+        constexpr int wo = (cxv(i) > 0) + G::DX * ((cyv(i) > 0) + G::DY * (czv(i) > 0));
+        s[(i - 1) * E + base + wo] = g[i];
     });
     __syncthreads();
     static_for<Q - 1>([&](auto I) {
         constexpr int i = decltype(I)::value + 1;
-        const int sx = (tx - cxv(i) + BX) % BX;
-        const int sy = (ty - cyv(i) + BY) % BY;
-        const int sz = (tz - czv(i) + BZ) % BZ;
-        pop[i] = s_pop[(i - 1) * T + sx + BX * (sy + BY * sz)];
+        constexpr int ro = (cxv(i) < 0) + G::DX * ((cyv(i) < 0) + G::DY * (czv(i) < 0));
+        g[i] = s[(i - 1) * E + base + ro];
     });
-
-    if (tx == 0)
-        get_face<Q, BX, BY, BZ, 1>(a, tx, ty, tz, bx, by, bz, pop);
-    else if (tx == BX - 1)
-        get_face<Q, BX, BY, BZ, 0>(a, tx, ty, tz, bx, by, bz, pop);
-    if (ty == 0)
-        get_face<Q, BX, BY, BZ, 3>(a, tx, ty, tz, bx, by, bz, pop);
-    else if (ty == BY - 1)
-        get_face<Q, BX, BY, BZ, 2>(a, tx, ty, tz, bx, by, bz, pop);
-    if (tz == 0)
-        get_face<Q, BX, BY, BZ, 5>(a, tx, ty, tz, bx, by, bz, pop);
-    else if (tz == BZ - 1)
-        get_face<Q, BX, BY, BZ, 4>(a, tx, ty, tz, bx, by, bz, pop);
 
     Node m;
     if (HAS_BC && cls != 0u && cls != 255u) {
-        bc_solve<Q>(a.bc + (size_t)cls * BC_STRIDE, pop, a.rho0, m);
+        bc_solve<Q>(a.bc + (size_t)cls * BC_STRIDE, g, a.rho0, m);
         scale_moments(m);
     } else {
-        bulk_moments<Q>(pop, a.rho0, m);
+        bulk_moments<Q>(g, a.rho0, m);
     }
     collide(m, a.R);
-    reconstruct<Q>(m, pop);
+    reconstruct<Q>(m, g);
+    if (!solid) store_node<Q, BX, BY, BZ, S>(a, mb, m);
 
-    if (!solid) {
-        store_node<Q, BX, BY, BZ, S>(a, mb, m);
-        if (tx == 0 && (!a.wall_x || x != 0)) put_face_c<Q, BX, BY, BZ, 0>(a.gout[0], ty, tz, tile, pop);
-        if (tx == BX - 1 && (!a.wall_x || x != a.nx - 1)) put_face_c<Q, BX, BY, BZ, 1>(a.gout[1], ty, tz, tile, pop);
-        if (ty == 0 && (!a.wall_y || y != 0)) put_face_c<Q, BX, BY, BZ, 2>(a.gout[2], tx, tz, tile, pop);
-        if (ty == BY - 1 && (!a.wall_y || y != a.ny - 1)) put_face_c<Q, BX, BY, BZ, 3>(a.gout[3], tx, tz, tile, pop);
-        if (tz == 0 && (!a.wall_z || z != 0)) put_face_c<Q, BX, BY, BZ, 4>(a.gout[4], tx, ty, tile, pop);
-        if (tz == BZ - 1 && (!a.wall_z || z != a.nz - 1)) put_face_c<Q, BX, BY, BZ, 5>(a.gout[5], tx, ty, tile, pop);
-    }
+    // populations leaving the tile: stage in LDS, then whole waves write
+    // the ghost planes (popSave.inc) in contiguous runs
+    __syncthreads();
+    static_for<Q - 1>([&](auto I) {
+        constexpr int i = decltype(I)::value + 1;
+        constexpr int wo = (cxv(i) > 0) + G::DX * ((cyv(i) > 0) + G::DY * (czv(i) > 0));
+        s[(i - 1) * E + base + wo] = g[i];
+    });
+    __syncthreads();
+    static_for<NW>([&](auto WW) {
+        constexpr int W = decltype(WW)::value;
+        if (w == W) {
+            static_for<(NOUT + NW - 1) / NW>([&](auto RR) {
+                constexpr int K = W + NW * decltype(RR)::value;
+                if constexpr (K < NOUT) halo_store<Q, BX, BY, BZ, HAS_BC, K>(a, lane, tile, bx, by, bz, s, s_cls);
+            });
+        }
+    });
 }
 
 // Initial ghost planes: every tile-face node writes its outgoing
