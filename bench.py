@@ -59,6 +59,35 @@ def cpu_baseline(target_s):
     }
 
 
+def reduced_precision_variants(n, tau, u0, steps, warmup):
+    """Same workload with FP16 and BF16 moment storage (FP32 arithmetic)."""
+    import mrlbm
+
+    res = {}
+    for name, st in (("fp16", mrlbm.STORE_FP16), ("bf16", mrlbm.STORE_BF16)):
+        lat = mrlbm.Lattice((n, n, n), q=19, tau=tau, storage=st)
+        lat.init_taylor_green(u0)
+        info = lat.info()
+        lat.step(warmup)
+        lat.synchronize()
+        t0 = time.perf_counter()
+        ev_ms = lat.time_steps(steps)
+        lat.synchronize()
+        t1 = time.perf_counter()
+        lat.close()
+        avg = ev_ms * 1e-3 / steps
+        res[name] = {
+            "value": round(n ** 3 * steps / (t1 - t0) / 1e6, 1),
+            "unit": "MLUPS",
+            "steps": steps,
+            "tile": [info["bx"], info["by"], info["bz"]],
+            "bytes_per_lu": info["bytes_per_lu"],
+            "mean_launch_us": round(avg * 1e6, 2),
+            "achieved_gbs": round(info["bytes_per_lu"] * n ** 3 / avg / 1e9, 1),
+        }
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +98,7 @@ def main():
     ap.add_argument("--kernel", default="auto", choices=["auto", "tile", "column"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true", help="skip the FP16/BF16 storage lines")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -147,6 +177,8 @@ def main():
             "traffic": traffic,
         },
     }
+    if not args.no_variants:
+        out["variants"] = reduced_precision_variants(n, tau, u0, args.steps // 2, args.warmup)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

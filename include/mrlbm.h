@@ -33,7 +33,10 @@ extern "C" {
 #define MRLBM_ESTATE     -5  /* call not valid in the current state */
 #define MRLBM_ENUMERIC   -6  /* singular boundary system, NaN detected */
 
-/* moment storage precision (arithmetic is always FP32) */
+/* moment storage precision (arithmetic is always FP32). FP16 stores
+ * value * 2^k (k = 12 for rho-RHO_0 and the second moments, 10 for U),
+ * BF16 stores the value; both round stochastically (counter-based bits
+ * keyed by node, step and seed). Ghost planes stay FP32. */
 #define MRLBM_STORE_FP32  0
 #define MRLBM_STORE_FP16  1
 #define MRLBM_STORE_BF16  2
@@ -63,6 +66,7 @@ typedef struct {
     double rho0;           /* RHO_0                                             */
     double fx, fy, fz;     /* body force L_Fx/L_Fy/L_Fz (src/mlbm.cu:76-78)     */
     int device;            /* HIP device ordinal                                */
+    unsigned int seed;     /* key of the stochastic rounding of FP16/BF16 storage */
 } mrlbm_params;
 
 typedef struct {

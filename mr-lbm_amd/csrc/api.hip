@@ -37,6 +37,24 @@ int fail(int code, const char* fmt, ...) {
 
 size_t storage_bytes(int storage) { return storage == MRLBM_STORE_FP32 ? 4 : 2; }
 
+// host-side storage conversion (round to nearest even; the kernels round
+// stochastically)
+uint16_t to_half(double v) { return __builtin_bit_cast(uint16_t, (_Float16)(float)v); }
+double from_half(uint16_t b) { return (double)(float)__builtin_bit_cast(_Float16, b); }
+uint16_t to_bf16(double v) {
+    const float f = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+double from_bf16(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return (double)f;
+}
+
 }  // namespace
 
 struct mrlbm_ctx {
@@ -59,6 +77,7 @@ struct mrlbm_ctx {
     int red_blocks = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double device_bytes = 0;
+    float mscale[10], mrscale[10];  // storage scales (powers of two)
 
     size_t mom_index(int x, int y, int z, int m) const {
         const int tile = (x / bx) + ntx * ((y / by) + nty * (z / bz));
@@ -101,7 +120,12 @@ struct mrlbm_ctx {
         a.R.fy = (float)p.fy;
         a.R.fz = (float)p.fz;
         a.R.has_force = (p.fx != 0.0 || p.fy != 0.0 || p.fz != 0.0) ? 1 : 0;
-        for (int m = 0; m < 10; m++) a.mscale[m] = a.mrscale[m] = 1.0f;
+        for (int m = 0; m < 10; m++) {
+            a.mscale[m] = mscale[m];
+            a.mrscale[m] = mrscale[m];
+        }
+        a.step = (uint32_t)step;
+        a.seed = p.seed;
         return a;
     }
     double bytes_per_lu() const {
@@ -156,6 +180,9 @@ int mrlbm_create(const mrlbm_params* pp, mrlbm_ctx** out) {
             const bool col = cand[c][0] * cand[c][1] == 256;
             if (kernel == MRLBM_KERNEL_TILE && col) continue;
             if (kernel == MRLBM_KERNEL_COLUMN && !col) continue;
+            // the column kernel is the default only where it is built and
+            // fastest: D3Q19 with FP32 storage
+            if (kernel == MRLBM_KERNEL_AUTO && col && (p.q != 19 || p.storage != MRLBM_STORE_FP32)) continue;
             if (p.nx % cand[c][0] == 0 && p.ny % cand[c][1] == 0 && p.nz % cand[c][2] == 0) {
                 bx = cand[c][0];
                 by = cand[c][1];
@@ -192,6 +219,16 @@ int mrlbm_create(const mrlbm_params* pp, mrlbm_ctx** out) {
     c->ntiles = c->ntx * c->nty * c->ntz;
     c->qf = p.q == 19 ? 5 : 9;
     c->N = (size_t)p.nx * p.ny * p.nz;
+    // storage scales: binary16 keeps the expected magnitudes (|drho|, |M| up
+    // to ~0.1, |U| up to ~0.3 for Mach < 0.2) near 2^8..2^9, far from both
+    // its overflow (65504) and its subnormal range (< 2^-14); bfloat16 has
+    // FP32's exponent range and needs none
+    for (int m = 0; m < 10; m++) {
+        float s = 1.0f;
+        if (p.storage == MRLBM_STORE_FP16) s = (m >= 1 && m <= 3) ? 1024.0f : 4096.0f;
+        c->mscale[m] = s;
+        c->mrscale[m] = 1.0f / s;
+    }
 
     auto alloc = [&](void** ptr, size_t bytes) -> int {
         hipError_t e = hipMalloc(ptr, bytes);
@@ -319,10 +356,51 @@ void node_moments(int q, int mode, double rho0, double rho, double ux, double uy
     m[9] = 4.5 * pzz;
 }
 
-int upload_and_ghosts(mrlbm_ctx* c, const std::vector<float>& blk) {
+// Moments (block layout, doubles, stored convention) -> storage bytes
+std::vector<char> encode_block(const mrlbm_ctx* c, const std::vector<double>& blk) {
+    const size_t sb = storage_bytes(c->p.storage);
+    std::vector<char> raw(blk.size() * sb);
+    const size_t T = c->T;
+    for (size_t j = 0; j < blk.size(); j++) {
+        const int m = (int)((j / T) % 10);
+        const double v = blk[j] * c->mscale[m];
+        if (c->p.storage == MRLBM_STORE_FP32) {
+            const float f = (float)v;
+            std::memcpy(&raw[j * 4], &f, 4);
+        } else {
+            const uint16_t h = c->p.storage == MRLBM_STORE_FP16 ? to_half(v) : to_bf16(v);
+            std::memcpy(&raw[j * 2], &h, 2);
+        }
+    }
+    return raw;
+}
+
+std::vector<double> decode_block(const mrlbm_ctx* c, const std::vector<char>& raw) {
+    const size_t sb = storage_bytes(c->p.storage);
+    std::vector<double> blk(raw.size() / sb);
+    const size_t T = c->T;
+    for (size_t j = 0; j < blk.size(); j++) {
+        const int m = (int)((j / T) % 10);
+        double v;
+        if (c->p.storage == MRLBM_STORE_FP32) {
+            float f;
+            std::memcpy(&f, &raw[j * 4], 4);
+            v = f;
+        } else {
+            uint16_t h;
+            std::memcpy(&h, &raw[j * 2], 2);
+            v = c->p.storage == MRLBM_STORE_FP16 ? from_half(h) : from_bf16(h);
+        }
+        blk[j] = v * (double)c->mrscale[m];
+    }
+    return blk;
+}
+
+int upload_and_ghosts(mrlbm_ctx* c, const std::vector<double>& blk) {
     HIPCHK(hipSetDevice(c->p.device));
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(c->d_mom, blk.data(), blk.size() * sizeof(float), hipMemcpyHostToDevice));
+    const std::vector<char> raw = encode_block(c, blk);
+    HIPCHK(hipMemcpy(c->d_mom, raw.data(), raw.size(), hipMemcpyHostToDevice));
     StepArgs a = c->args();
     for (int s = 0; s < 2; s++) {
         if (c->p.kernel == MRLBM_KERNEL_COLUMN)
@@ -343,8 +421,7 @@ template <typename F>
 int init_with(mrlbm_ctx* c, int mode, F&& field) {
     if (mode != MRLBM_INIT_EQUILIBRIUM && mode != MRLBM_INIT_REFERENCE)
         return fail(MRLBM_EINVAL, "unknown second-moment mode %d", mode);
-    if (c->p.storage != MRLBM_STORE_FP32) return fail(MRLBM_EINVAL, "reduced-precision storage not built yet");
-    std::vector<float> blk(c->N * 10);
+    std::vector<double> blk(c->N * 10);
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
     for (int z = 0; z < nz; z++)
         for (int y = 0; y < ny; y++)
@@ -353,7 +430,7 @@ int init_with(mrlbm_ctx* c, int mode, F&& field) {
                 field(x, y, z, r, ux, uy, uz);
                 if (!(r > 0)) return fail(MRLBM_EINVAL, "non-positive density at (%d,%d,%d)", x, y, z);
                 node_moments(c->p.q, mode, c->p.rho0, r, ux, uy, uz, m);
-                for (int k = 0; k < 10; k++) blk[c->mom_index(x, y, z, k)] = (float)m[k];
+                for (int k = 0; k < 10; k++) blk[c->mom_index(x, y, z, k)] = m[k];
             }
     return upload_and_ghosts(c, blk);
 }
@@ -445,11 +522,11 @@ int mrlbm_synchronize(mrlbm_ctx* c) {
 
 int mrlbm_read_moments(mrlbm_ctx* c, double* out) {
     if (!c || !out) return fail(MRLBM_EINVAL, "null argument");
-    if (c->p.storage != MRLBM_STORE_FP32) return fail(MRLBM_EINVAL, "reduced-precision storage not built yet");
     HIPCHK(hipSetDevice(c->p.device));
     HIPCHK(hipStreamSynchronize(c->stream));
-    std::vector<float> blk(c->N * 10);
-    HIPCHK(hipMemcpy(blk.data(), c->d_mom, blk.size() * sizeof(float), hipMemcpyDeviceToHost));
+    std::vector<char> raw(c->N * 10 * storage_bytes(c->p.storage));
+    HIPCHK(hipMemcpy(raw.data(), c->d_mom, raw.size(), hipMemcpyDeviceToHost));
+    const std::vector<double> blk = decode_block(c, raw);
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
     for (int k = 0; k < 10; k++)
         for (int z = 0; z < nz; z++)
@@ -461,22 +538,21 @@ int mrlbm_read_moments(mrlbm_ctx* c, double* out) {
 
 int mrlbm_write_moments(mrlbm_ctx* c, const double* in) {
     if (!c || !in) return fail(MRLBM_EINVAL, "null argument");
-    if (c->p.storage != MRLBM_STORE_FP32) return fail(MRLBM_EINVAL, "reduced-precision storage not built yet");
-    std::vector<float> blk(c->N * 10);
+    std::vector<double> blk(c->N * 10);
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
     for (int k = 0; k < 10; k++)
         for (int z = 0; z < nz; z++)
             for (int y = 0; y < ny; y++)
                 for (int x = 0; x < nx; x++)
-                    blk[c->mom_index(x, y, z, k)] = (float)in[(size_t)k * c->N + x + (size_t)nx * (y + (size_t)ny * z)];
+                    blk[c->mom_index(x, y, z, k)] = in[(size_t)k * c->N + x + (size_t)nx * (y + (size_t)ny * z)];
     return upload_and_ghosts(c, blk);
 }
 
 int mrlbm_reduce(mrlbm_ctx* c, double* mass, double* ke) {
     if (!c || !mass || !ke) return fail(MRLBM_EINVAL, "null argument");
     HIPCHK(hipSetDevice(c->p.device));
-    HIPCHK(launch_reduce(c->p.storage, c->d_mom, c->ntiles, c->T, (float)c->p.rho0, c->d_partial, c->red_blocks,
-                         c->d_red, c->stream));
+    HIPCHK(launch_reduce(c->p.storage, c->d_mom, c->ntiles, c->T, (float)c->p.rho0, c->mrscale[0], c->mrscale[1],
+                         c->d_partial, c->red_blocks, c->d_red, c->stream));
     double h[2];
     HIPCHK(hipMemcpyAsync(h, c->d_red, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

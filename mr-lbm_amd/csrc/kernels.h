@@ -9,15 +9,55 @@
 
 namespace mrlbm {
 
-// Storage codecs. Arithmetic is always FP32; a codec only maps a stored
-// value to/from float. `enc` holds per-moment scale factors for the
-// reduced-precision codecs (power-of-two, so scaling is exact).
+// Storage codecs. Arithmetic is always FP32; a codec maps a stored value
+// to/from float. Reduced-precision codecs store v * scale (a power of two,
+// so the scaling is exact) and round stochastically: `rnd` carries uniform
+// random bits (one draw per value, counter-based, see sr_bits below); adding
+// them below the kept mantissa bits and truncating gives an unbiased
+// rounding, so per-step changes smaller than half an ulp are not lost.
 template <int S> struct Codec;
 template <> struct Codec<0> {
     using T = float;
     __device__ __forceinline__ static float ld(const T* p, size_t i, float) { return p[i]; }
     __device__ __forceinline__ static void st(T* p, size_t i, float v, float) { p[i] = v; }
+    __device__ __forceinline__ static T enc(float v, float, uint32_t) { return v; }
+    __device__ __forceinline__ static float dec(T b, float) { return b; }
 };
+template <> struct Codec<1> {  // IEEE binary16, 13 dropped bits rounded stochastically
+    using T = uint16_t;
+    __device__ __forceinline__ static float dec(T b, float rscale) {
+        return (float)__builtin_bit_cast(_Float16, b) * rscale;
+    }
+    __device__ __forceinline__ static T enc(float v, float scale, uint32_t rnd) {
+        const uint32_t u = __float_as_uint(v * scale);
+        const float t = __uint_as_float((u + (rnd & 0x1FFFu)) & ~0x1FFFu);
+        return __builtin_bit_cast(T, (_Float16)t);  // exact for normal binary16 values
+    }
+    __device__ __forceinline__ static float ld(const T* p, size_t i, float rscale) { return dec(p[i], rscale); }
+};
+template <> struct Codec<2> {  // bfloat16, 16 dropped bits rounded stochastically
+    using T = uint16_t;
+    __device__ __forceinline__ static float dec(T b, float rscale) { return __uint_as_float((uint32_t)b << 16) * rscale; }
+    __device__ __forceinline__ static T enc(float v, float scale, uint32_t rnd) {
+        const uint32_t u = __float_as_uint(v * scale);
+        return (T)((u + (rnd & 0xFFFFu)) >> 16);
+    }
+    __device__ __forceinline__ static float ld(const T* p, size_t i, float rscale) { return dec(p[i], rscale); }
+};
+
+// Counter-based random bits for stochastic rounding: a 32-bit mix of the
+// node's global index, the time step and the lattice seed. The same node,
+// step and seed give the same bits in any launch shape or batch, so results
+// do not depend on how lattices are batched.
+__device__ __forceinline__ uint32_t sr_bits(uint32_t node, uint32_t step, uint32_t seed) {
+    uint32_t x = node ^ (step * 0x9E3779B9u) ^ seed;
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
 
 // Arguments of one lattice step. Ghost faces: index 0 X_0, 1 X_1, 2 Y_0,
 // 3 Y_1, 4 Z_0, 5 Z_1. `gin` is read (previous step's outgoing
@@ -36,6 +76,8 @@ struct StepArgs {
     Relax R;
     float mscale[10];        // storage scale per moment (reduced precision)
     float mrscale[10];       // inverse scale
+    uint32_t step;           // time step being computed (stochastic rounding key)
+    uint32_t seed;           // lattice seed (stochastic rounding key)
 };
 
 // Ghost-plane indices of the reference layout with runtime tile shape
@@ -75,7 +117,7 @@ hipError_t launch_init_ghosts(int q, int bx, int by, int bz, int storage, const 
 
 // FP64 partial sums of (rho - RHO_0) and rho |u|^2 / 2 per block; then one
 // block folds the partials. Deterministic (no atomics).
-hipError_t launch_reduce(int storage, const void* mom, size_t ntiles, int tile_nodes, float rho0, double* partial,
-                         int nblocks, double* out2, hipStream_t s);
+hipError_t launch_reduce(int storage, const void* mom, size_t ntiles, int tile_nodes, float rho0, float rs0, float rs1,
+                         double* partial, int nblocks, double* out2, hipStream_t s);
 
 }  // namespace mrlbm

@@ -65,11 +65,19 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double* part
     }
 }
 
-hipError_t launch_reduce(int storage, const void* mom, size_t ntiles, int tile_nodes, float rho0, double* partial,
-                         int nblocks, double* out2, hipStream_t s) {
-    if (storage != 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_reduce_partial<0>, dim3(nblocks), dim3(RED_THREADS), 0, s, mom, ntiles, tile_nodes, rho0,
-                       1.0f, 1.0f, partial);
+hipError_t launch_reduce(int storage, const void* mom, size_t ntiles, int tile_nodes, float rho0, float rs0, float rs1,
+                         double* partial, int nblocks, double* out2, hipStream_t s) {
+    if (storage == 0)
+        hipLaunchKernelGGL(k_reduce_partial<0>, dim3(nblocks), dim3(RED_THREADS), 0, s, mom, ntiles, tile_nodes, rho0,
+                           rs0, rs1, partial);
+    else if (storage == 1)
+        hipLaunchKernelGGL(k_reduce_partial<1>, dim3(nblocks), dim3(RED_THREADS), 0, s, mom, ntiles, tile_nodes, rho0,
+                           rs0, rs1, partial);
+    else if (storage == 2)
+        hipLaunchKernelGGL(k_reduce_partial<2>, dim3(nblocks), dim3(RED_THREADS), 0, s, mom, ntiles, tile_nodes, rho0,
+                           rs0, rs1, partial);
+    else
+        return hipErrorInvalidValue;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, s, partial, nblocks, out2);

@@ -141,13 +141,15 @@ __global__ __launch_bounds__(BX* BY) void k_step_column(StepArgs a) {
             bulk_moments<Q>(in0, a.rho0, m);
         }
         collide(m, a.R);
+        const int z = bz * BZ + k;
+        typename Codec<S>::T qv[10];
+        quantize<S>(m, a, (uint32_t)x + (uint32_t)a.nx * ((uint32_t)y + (uint32_t)a.ny * (uint32_t)z), qv);
         float g[Q];
         reconstruct<Q>(m, g);
 
         const bool solid = HAS_BC && (cls == 255u);
         if (!solid) {
-            const int z = bz * BZ + k;
-            store_node<Q, BX, BY, BZ, S>(a, mbase + (size_t)k * P, m);
+            store_node<Q, BX, BY, BZ, S>(a, mbase + (size_t)k * P, qv);
             if (tx == 0 && (!a.wall_x || x != 0)) put_face_c<Q, BX, BY, BZ, 0>(a.gout[0], ty, k, tile, g);
             if (tx == BX - 1 && (!a.wall_x || x != a.nx - 1)) put_face_c<Q, BX, BY, BZ, 1>(a.gout[1], ty, k, tile, g);
             if (ty == 0 && (!a.wall_y || y != 0)) put_face_c<Q, BX, BY, BZ, 2>(a.gout[2], tx, k, tile, g);

@@ -154,8 +154,11 @@ __global__ __launch_bounds__(BX* BY* BZ) void k_step_tile(StepArgs a) {
         bulk_moments<Q>(g, a.rho0, m);
     }
     collide(m, a.R);
+    typename Codec<S>::T q[10];
+    const uint32_t node_id = (uint32_t)(bx * BX + tx) + (uint32_t)a.nx * ((uint32_t)(by * BY + ty) + (uint32_t)a.ny * (uint32_t)(bz * BZ + tz));
+    quantize<S>(m, a, node_id, q);
     reconstruct<Q>(m, g);
-    if (!solid) store_node<Q, BX, BY, BZ, S>(a, mb, m);
+    if (!solid) store_node<Q, BX, BY, BZ, S>(a, mb, q);
 
     // populations leaving the tile: stage in LDS, then whole waves write
     // the ghost planes (popSave.inc) in contiguous runs
@@ -208,7 +211,7 @@ __global__ __launch_bounds__(BX* BY* BZ) void k_init_ghosts(StepArgs a, GhostOut
 
 bool tile_shape_supported(int q, int bx, int by, int bz, int storage) {
     if (q != 19 && q != 27) return false;
-    if (storage != 0) return false;
+    if (storage < 0 || storage > 2) return false;
 #define X(A, B, C) \
     if (bx == A && by == B && bz == C) return true;
     MRLBM_TILE_SHAPES(X)
@@ -216,16 +219,21 @@ bool tile_shape_supported(int q, int bx, int by, int bz, int storage) {
     return false;
 }
 
+template <int Q, int BX, int BY, int BZ, int S>
+static hipError_t launch_qs(bool has_bc, const StepArgs& a, int ntiles, hipStream_t s) {
+    dim3 grid(ntiles), block(BX * BY * BZ);
+    if (has_bc)
+        hipLaunchKernelGGL((k_step_tile<Q, BX, BY, BZ, true, S>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((k_step_tile<Q, BX, BY, BZ, false, S>), grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
 template <int Q, int BX, int BY, int BZ>
 static hipError_t launch_q_shape(bool has_bc, int storage, const StepArgs& a, int ntiles, hipStream_t s) {
-    dim3 grid(ntiles), block(BX * BY * BZ);
-    if (storage == 0) {
-        if (has_bc)
-            hipLaunchKernelGGL((k_step_tile<Q, BX, BY, BZ, true, 0>), grid, block, 0, s, a);
-        else
-            hipLaunchKernelGGL((k_step_tile<Q, BX, BY, BZ, false, 0>), grid, block, 0, s, a);
-        return hipGetLastError();
-    }
+    if (storage == 0) return launch_qs<Q, BX, BY, BZ, 0>(has_bc, a, ntiles, s);
+    if (storage == 1) return launch_qs<Q, BX, BY, BZ, 1>(has_bc, a, ntiles, s);
+    if (storage == 2) return launch_qs<Q, BX, BY, BZ, 2>(has_bc, a, ntiles, s);
     return hipErrorInvalidValue;
 }
 
@@ -241,17 +249,24 @@ hipError_t launch_step_tile(int q, int bx, int by, int bz, bool has_bc, int stor
     return hipErrorInvalidValue;
 }
 
+template <int Q, int BX, int BY, int BZ>
+static void launch_ig(int storage, const StepArgs& a, const GhostOut& go, int ntiles, hipStream_t s) {
+    dim3 grid(ntiles), block(BX * BY * BZ);
+    if (storage == 0) hipLaunchKernelGGL((k_init_ghosts<Q, BX, BY, BZ, 0>), grid, block, 0, s, a, go);
+    if (storage == 1) hipLaunchKernelGGL((k_init_ghosts<Q, BX, BY, BZ, 1>), grid, block, 0, s, a, go);
+    if (storage == 2) hipLaunchKernelGGL((k_init_ghosts<Q, BX, BY, BZ, 2>), grid, block, 0, s, a, go);
+}
+
 hipError_t launch_init_ghosts(int q, int bx, int by, int bz, int storage, const StepArgs& a, float* const g[6],
                               int ntiles, hipStream_t s) {
     GhostOut go;
     for (int f = 0; f < 6; f++) go.f[f] = g[f];
-    dim3 grid(ntiles), block(bx * by * bz);
-    if (storage != 0) return hipErrorInvalidValue;
-#define X(A, B, C)                                                                                   \
-    if (bx == A && by == B && bz == C) {                                                             \
-        if (q == 19) hipLaunchKernelGGL((k_init_ghosts<19, A, B, C, 0>), grid, block, 0, s, a, go); \
-        else hipLaunchKernelGGL((k_init_ghosts<27, A, B, C, 0>), grid, block, 0, s, a, go);         \
-        return hipGetLastError();                                                                    \
+    if (storage < 0 || storage > 2) return hipErrorInvalidValue;
+#define X(A, B, C)                                                   \
+    if (bx == A && by == B && bz == C) {                             \
+        if (q == 19) launch_ig<19, A, B, C>(storage, a, go, ntiles, s); \
+        else launch_ig<27, A, B, C>(storage, a, go, ntiles, s);         \
+        return hipGetLastError();                                    \
     }
     MRLBM_TILE_SHAPES(X)
 #undef X

@@ -42,21 +42,63 @@ __device__ __forceinline__ void load_node(const StepArgs& a, size_t mb, Node& n)
     n.mzz = C::ld(mom, mb + 9 * T, a.mrscale[9]);
 }
 
-template <int Q, int BX, int BY, int BZ, int S>
-__device__ __forceinline__ void store_node(const StepArgs& a, size_t mb, const Node& n) {
+// Round the post-collision moments to the storage format and replace them
+// by the values the storage will hold, so the populations sent through the
+// ghost planes are the ones the neighbour would reconstruct from the stored
+// moments: results do not depend on the tile shape. FP32 storage is exact.
+// Stochastic rounding draws 8 random bits per value from three words:
+// one counter-based hash (sr_bits) and two xorshift successors.
+__device__ __forceinline__ uint32_t xs32(uint32_t x) {
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x;
+}
+
+template <int S>
+__device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t node_id, typename Codec<S>::T* q) {
     using C = Codec<S>;
+    if constexpr (S == 0) {
+        q[0] = n.drho;
+        q[1] = n.ux;
+        q[2] = n.uy;
+        q[3] = n.uz;
+        q[4] = n.mxx;
+        q[5] = n.mxy;
+        q[6] = n.mxz;
+        q[7] = n.myy;
+        q[8] = n.myz;
+        q[9] = n.mzz;
+    } else {
+        const uint32_t w0 = sr_bits(node_id, a.step, a.seed);
+        const uint32_t w1 = xs32(w0), w2 = xs32(w1);
+        constexpr uint32_t SH = S == 1 ? 5u : 8u;            // random byte -> dropped-bit position
+        constexpr uint32_t HALF = S == 1 ? 16u : 128u;       // centre of the byte's interval
+        auto rb = [&](uint32_t w, int k) { return (((w >> (8 * k)) & 0xFFu) << SH) | HALF; };
+#define MRLBM_Q(K, FIELD, W, B)                          \
+    q[K] = C::enc(n.FIELD, a.mscale[K], rb(W, B));       \
+    n.FIELD = C::dec(q[K], a.mrscale[K]);
+        MRLBM_Q(0, drho, w0, 0)
+        MRLBM_Q(1, ux, w0, 1)
+        MRLBM_Q(2, uy, w0, 2)
+        MRLBM_Q(3, uz, w0, 3)
+        MRLBM_Q(4, mxx, w1, 0)
+        MRLBM_Q(5, mxy, w1, 1)
+        MRLBM_Q(6, mxz, w1, 2)
+        MRLBM_Q(7, myy, w1, 3)
+        MRLBM_Q(8, myz, w2, 0)
+        MRLBM_Q(9, mzz, w2, 1)
+#undef MRLBM_Q
+        n.rho = a.rho0 + n.drho;
+    }
+}
+
+template <int Q, int BX, int BY, int BZ, int S>
+__device__ __forceinline__ void store_node(const StepArgs& a, size_t mb, const typename Codec<S>::T* q) {
     constexpr size_t T = (size_t)BX * BY * BZ;
-    typename C::T* mom = (typename C::T*)a.mom;
-    C::st(mom, mb + 0 * T, n.drho, a.mscale[0]);
-    C::st(mom, mb + 1 * T, n.ux, a.mscale[1]);
-    C::st(mom, mb + 2 * T, n.uy, a.mscale[2]);
-    C::st(mom, mb + 3 * T, n.uz, a.mscale[3]);
-    C::st(mom, mb + 4 * T, n.mxx, a.mscale[4]);
-    C::st(mom, mb + 5 * T, n.mxy, a.mscale[5]);
-    C::st(mom, mb + 6 * T, n.mxz, a.mscale[6]);
-    C::st(mom, mb + 7 * T, n.myy, a.mscale[7]);
-    C::st(mom, mb + 8 * T, n.myz, a.mscale[8]);
-    C::st(mom, mb + 9 * T, n.mzz, a.mscale[9]);
+    typename Codec<S>::T* mom = (typename Codec<S>::T*)a.mom;
+#pragma unroll
+    for (int k = 0; k < 10; k++) mom[mb + k * T] = q[k];
 }
 
 // Write the outgoing populations of face FACE of node (u, v) of the face.

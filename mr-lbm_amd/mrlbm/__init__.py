@@ -39,12 +39,12 @@ class Lattice:
     """
 
     def __init__(self, shape, q=19, tau=0.6, rho0=1.0, walls=(0, 0, 0), force=(0.0, 0.0, 0.0), tile=None,
-                 storage=STORE_FP32, kernel=KERNEL_AUTO, device=0):
+                 storage=STORE_FP32, kernel=KERNEL_AUTO, device=0, seed=0):
         L = _lib.load()
         nx, ny, nz = shape
         bx, by, bz = tile if tile else (0, 0, 0)
         p = _lib.Params(nx, ny, nz, q, bx, by, bz, storage, walls[0], walls[1], walls[2], kernel, tau, rho0,
-                        force[0], force[1], force[2], device)
+                        force[0], force[1], force[2], device, seed)
         h = ctypes.c_void_p()
         _lib.check(L.mrlbm_create(ctypes.byref(p), ctypes.byref(h)))
         self._h = h

@@ -22,7 +22,7 @@ class Params(ctypes.Structure):
         ("wall_x", ctypes.c_int), ("wall_y", ctypes.c_int), ("wall_z", ctypes.c_int), ("kernel", ctypes.c_int),
         ("tau", ctypes.c_double), ("rho0", ctypes.c_double),
         ("fx", ctypes.c_double), ("fy", ctypes.c_double), ("fz", ctypes.c_double),
-        ("device", ctypes.c_int),
+        ("device", ctypes.c_int), ("seed", ctypes.c_uint),
     ]
 
 

@@ -233,3 +233,82 @@ def test_column_kernel_cavity_and_spheres(q):
     e = rel_l2(res[1][:, fluid], orc.moments()[:, fluid])
     say("rel L2", e)
     assert max(e.values()) <= TOL
+
+
+@pytest.mark.parametrize("storage,bar", [(mrlbm.STORE_FP16, 1e-3), (mrlbm.STORE_BF16, 1e-2)])
+def test_reduced_precision_taylor_green(storage, bar):
+    """FP16/BF16 moment storage with FP32 arithmetic vs the FP64 oracle."""
+    n, steps, tau = 64, 500, 0.53
+    with mrlbm.Lattice((n, n, n), q=19, tau=tau, storage=storage, tile=(8, 8, 8)) as lat:
+        lat.init_taylor_green(0.05)
+        lat.step(steps)
+        g = lat.moments()
+        gm, gk = lat.reduce()
+    orc = O.Oracle(n, n, n, q=19, tau=tau)
+    orc.init_taylor_green(0.05)
+    orc.step(steps)
+    e = rel_l2(g, orc.moments())
+    om_, ok = orc.reduce()
+    say("storage", storage, "rel L2", e, "ke", gk, ok, "mass", gm, om_)
+    assert e["rho"] <= bar and e["u"] <= bar
+    assert abs(gk / ok - 1) <= bar
+
+
+@pytest.mark.parametrize("storage", [mrlbm.STORE_FP16, mrlbm.STORE_BF16])
+def test_reduced_precision_tile_independent(storage):
+    """Ghost populations come from the rounded moments, and the rounding is
+    keyed by node, step and seed: any tile shape gives the same bits."""
+    n = 64
+    ref = None
+    for tile in [(8, 8, 8), (16, 8, 4), (64, 4, 2)]:
+        with mrlbm.Lattice((n, n, n), q=19, tau=0.55, tile=tile, storage=storage, seed=7) as lat:
+            lat.init_taylor_green(0.05)
+            lat.step(40)
+            m = lat.moments()
+        if ref is None:
+            ref = m
+        else:
+            assert np.array_equal(m, ref), tile
+
+
+def test_fp16_slow_decay_not_frozen():
+    """TGV at nu = 0.01 decays ~2e-5 per step in relative terms, below FP16's
+    half-ulp: round-to-nearest storage would freeze it; stochastic rounding
+    must follow the FP64 decay."""
+    n, steps, tau = 32, 2000, 0.53
+    with mrlbm.Lattice((n, n, n), q=19, tau=tau, storage=mrlbm.STORE_FP16) as lat:
+        lat.init_taylor_green(0.02)
+        _, k0 = lat.reduce()
+        lat.step(steps)
+        _, k1 = lat.reduce()
+    orc = O.Oracle(n, n, n, q=19, tau=tau)
+    orc.init_taylor_green(0.02)
+    _, o0 = orc.reduce()
+    orc.step(steps)
+    _, o1 = orc.reduce()
+    say("fp16 decay", k1 / k0, "fp64", o1 / o0)
+    assert abs((k1 / k0) - (o1 / o0)) < 0.05 * (1 - o1 / o0)
+
+
+@pytest.mark.parametrize("storage", [mrlbm.STORE_FP16, mrlbm.STORE_BF16])
+def test_reduced_precision_forced_channel(storage):
+    """Force 1e-6 per step, below half an ulp of the developing velocity:
+    the steady parabola must still be reached (stochastic rounding)."""
+    ny, nu, F = 17, 1.0 / 6.0, 1e-6
+    shape = (8, ny + 15, 8)  # wall nodes at y = 0 and y = ny-1; the rest solid
+    types = np.zeros(shape, np.uint32)
+    types[:, 0, :] = 0x33
+    types[:, ny - 1, :] = 0xCC
+    types[:, ny:, :] = 0xFF
+    with mrlbm.Lattice(shape, q=19, tau=0.5 + 3 * nu, walls=(0, 1, 0), force=(F, 0, 0), storage=storage,
+                       tile=(8, 8, 8)) as lat:
+        lat.set_node_types(types)
+        lat.init_uniform()
+        lat.step(6000)
+        m = lat.moments()
+    ux = (m[1] / 3).reshape(shape, order="F")[:, :ny, :].mean(axis=(0, 2))
+    y = np.arange(ny)
+    ana = F * y * (ny - 1 - y) / (2 * nu) + F
+    err = np.linalg.norm(ux - ana) / np.linalg.norm(ana)
+    say("storage", storage, "channel rel err", err)
+    assert err < (2e-3 if storage == mrlbm.STORE_FP16 else 2e-2)
