@@ -159,6 +159,40 @@ int mrlbm_get_info(mrlbm_ctx* ctx, mrlbm_info* out);
 int mrlbm_bc_class_info(int q, uint32_t node_type, double omega, const double u[3], uint32_t* known,
                         double* ainv49);
 
+/* ---------------------------------------------------------------------
+ * Batches (no reference counterpart: the reference steps one lattice per
+ * launch with a device sync per step, src/main.cu:223-230). A batch
+ * advances many lattices of the same type (q, tile shape, storage, device,
+ * tile kernel) with one launch per step through a block -> (lattice, tile)
+ * work table. Members are ordinary contexts: they adopt the batch's stream,
+ * keep their own parameters, node types and state, and are read back,
+ * reduced, checkpointed or re-initialised with the mrlbm_* calls above;
+ * mrlbm_step on a member is refused. Results are bit-identical to stepping
+ * each lattice alone.
+ */
+typedef struct mrlbm_batch mrlbm_batch;
+int mrlbm_batch_create(int device, int q, int bx, int by, int bz, int storage, mrlbm_batch** out);
+int mrlbm_batch_add(mrlbm_batch* b, mrlbm_ctx* ctx);
+int mrlbm_batch_remove(mrlbm_batch* b, mrlbm_ctx* ctx);
+/* 0 (default): one workgroup per work item; n > 0: at most n workgroups
+ * walking the work table (persistent launch) */
+int mrlbm_batch_set_grid(mrlbm_batch* b, int max_blocks);
+/* Advance every member by nsteps. graph_steps = 0: plain launches;
+ * graph_steps = g > 0: a hipGraph of g steps is captured (once per
+ * membership) and replayed nsteps/g times (nsteps must be a multiple). */
+int mrlbm_batch_step(mrlbm_batch* b, int nsteps, int graph_steps);
+/* Same, bracketed by HIP events on the batch stream; *elapsed_ms. */
+int mrlbm_batch_time_steps(mrlbm_batch* b, int nsteps, int graph_steps, double* elapsed_ms);
+int mrlbm_batch_synchronize(mrlbm_batch* b);
+int mrlbm_batch_size(mrlbm_batch* b, int* members, int64_t* nodes);
+int mrlbm_batch_destroy(mrlbm_batch* b);
+/* Autotuner: for each candidate tile shape (shapes = 3*nshapes ints) that
+ * divides every lattice extent (dims = 3*ninst ints), time `steps` batched
+ * graph steps of all-bulk lattices; *best = index of the fastest,
+ * ms_out[s] = its time (-1 if the shape does not fit). */
+int mrlbm_batch_autotune(int device, int q, int storage, int nshapes, const int* shapes, int ninst, const int* dims,
+                         int steps, int* best, double* ms_out);
+
 /* Measurement utility (no reference counterpart): time `iters` plain
  * device-to-device copies of `bytes` on `device`; *ms_per_copy = mean event
  * time of one copy (it reads and writes `bytes` each). */

@@ -11,11 +11,12 @@
 
 #include "../../include/mrlbm.h"
 #include "bc_classes.h"
+#include "context.h"
 #include "kernels.h"
 
 using namespace mrlbm;
 
-namespace {
+namespace mrlbm {
 
 thread_local std::string g_err;
 
@@ -28,12 +29,6 @@ int fail(int code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
-
-#define HIPCHK(expr)                                                                          \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess) return fail(MRLBM_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
 
 size_t storage_bytes(int storage) { return storage == MRLBM_STORE_FP32 ? 4 : 2; }
 
@@ -55,87 +50,7 @@ double from_bf16(uint16_t b) {
     return (double)f;
 }
 
-}  // namespace
-
-struct mrlbm_ctx {
-    mrlbm_params p;
-    int bx, by, bz, T, ntx, nty, ntz, ntiles, qf;
-    size_t N;
-    hipStream_t stream = nullptr;
-    void* d_mom = nullptr;
-    uint8_t* d_cls = nullptr;
-    float* d_bc = nullptr;
-    float* d_ghost[2][6] = {};
-    size_t face_elems[6] = {};
-    int cur = 0;  // ghost set read by the next step
-    int64_t step = 0;
-    bool has_bc = false, has_solid = false, initialised = false;
-    int n_classes = 0;
-    double bc_vel[4][3] = {};
-    double* d_partial = nullptr;
-    double* d_red = nullptr;
-    int red_blocks = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double device_bytes = 0;
-    float mscale[10], mrscale[10];  // storage scales (powers of two)
-
-    size_t mom_index(int x, int y, int z, int m) const {
-        const int tile = (x / bx) + ntx * ((y / by) + nty * (z / bz));
-        const int t = (x % bx) + bx * ((y % by) + by * (z % bz));
-        return (size_t)tile * 10 * T + (size_t)m * T + t;
-    }
-    size_t node_index(int x, int y, int z) const {
-        const int tile = (x / bx) + ntx * ((y / by) + nty * (z / bz));
-        const int t = (x % bx) + bx * ((y % by) + by * (z % bz));
-        return (size_t)tile * T + t;
-    }
-    StepArgs args() const {
-        StepArgs a{};
-        a.mom = d_mom;
-        a.cls = d_cls;
-        a.bc = d_bc;
-        for (int f = 0; f < 6; f++) {
-            a.gin[f] = d_ghost[cur][f];
-            a.gout[f] = d_ghost[1 - cur][f];
-        }
-        a.ntx = ntx;
-        a.nty = nty;
-        a.ntz = ntz;
-        a.nx = p.nx;
-        a.ny = p.ny;
-        a.nz = p.nz;
-        a.wall_x = p.wall_x;
-        a.wall_y = p.wall_y;
-        a.wall_z = p.wall_z;
-        a.rho0 = (float)p.rho0;
-        const double omega = 1.0 / p.tau;
-        // relaxation constants as src/mlbm.cu:56-60 derives them
-        const float om = (float)omega;
-        a.R.omega = om;
-        a.R.t_omega = 1.0f - om;
-        a.R.tt_omega = 1.0f - om / 2.0f;
-        a.R.omega_d2 = om / 2.0f;
-        a.R.tt_omega_t3 = a.R.tt_omega * 3.0f;
-        a.R.fx = (float)p.fx;
-        a.R.fy = (float)p.fy;
-        a.R.fz = (float)p.fz;
-        a.R.has_force = (p.fx != 0.0 || p.fy != 0.0 || p.fz != 0.0) ? 1 : 0;
-        for (int m = 0; m < 10; m++) {
-            a.mscale[m] = mscale[m];
-            a.mrscale[m] = mrscale[m];
-        }
-        a.step = (uint32_t)step;
-        a.seed = p.seed;
-        return a;
-    }
-    double bytes_per_lu() const {
-        const double sm = (double)storage_bytes(p.storage);
-        const double sg = 4.0;
-        double b = 2.0 * 10.0 * sm + 4.0 * qf * sg * (1.0 / bx + 1.0 / by + 1.0 / bz);
-        if (has_bc) b += 1.0;
-        return b;
-    }
-};
+}  // namespace mrlbm
 
 extern "C" {
 
@@ -143,6 +58,7 @@ const char* mrlbm_last_error(void) { return g_err.c_str(); }
 
 int mrlbm_destroy(mrlbm_ctx* c) {
     if (!c) return MRLBM_OK;
+    if (c->batch) mrlbm_batch_remove(c->batch, c);
     hipSetDevice(c->p.device);
     if (c->stream) hipStreamSynchronize(c->stream);
     hipFree(c->d_mom);
@@ -294,6 +210,7 @@ int mrlbm_set_node_types(mrlbm_ctx* c, const uint32_t* types) {
     c->has_bc = tab.has_bc;
     c->has_solid = tab.has_solid;
     c->n_classes = (int)tab.keys.size() - 1;
+    c->version++;
     if (!c->has_bc) return MRLBM_OK;
     std::vector<uint8_t> blk(c->N);
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
@@ -414,6 +331,7 @@ int upload_and_ghosts(mrlbm_ctx* c, const std::vector<double>& blk) {
     c->cur = 0;
     c->step = 0;
     c->initialised = true;
+    c->version++;
     return MRLBM_OK;
 }
 
@@ -492,6 +410,7 @@ static int do_steps(mrlbm_ctx* c, int nsteps) {
 
 int mrlbm_step(mrlbm_ctx* c, int nsteps) {
     if (!c) return fail(MRLBM_EINVAL, "null context");
+    if (c->batch) return fail(MRLBM_ESTATE, "lattice is stepped by its batch (mrlbm_batch_step)");
     if (!c->initialised) return fail(MRLBM_ESTATE, "lattice not initialised");
     if (nsteps < 0) return fail(MRLBM_EINVAL, "negative step count");
     HIPCHK(hipSetDevice(c->p.device));
@@ -500,6 +419,7 @@ int mrlbm_step(mrlbm_ctx* c, int nsteps) {
 
 int mrlbm_time_steps(mrlbm_ctx* c, int nsteps, double* ms) {
     if (!c || !ms) return fail(MRLBM_EINVAL, "null argument");
+    if (c->batch) return fail(MRLBM_ESTATE, "lattice is stepped by its batch (mrlbm_batch_time_steps)");
     if (!c->initialised) return fail(MRLBM_ESTATE, "lattice not initialised");
     HIPCHK(hipSetDevice(c->p.device));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -684,6 +604,7 @@ int mrlbm_checkpoint_load(mrlbm_ctx* c, const char* path) {
     c->cur = 0;
     c->step = h.step;
     c->initialised = true;
+    c->version++;
     return MRLBM_OK;
 }
 

@@ -97,6 +97,25 @@ struct GhostOut {
     float* f[6];
 };
 
+// One lattice of a batch (device table entry). The kernel of launch offset
+// k reads ghost set (cur0 + k) & 1 and computes time step step0 + k; after
+// a sequence of n launches an advance kernel adds n to both.
+struct InstArgs {
+    StepArgs a;      // gin/gout/step are filled per launch
+    float* g[2][6];  // both ghost sets
+    int cur0;
+    uint32_t step0;
+    int has_bc;
+    int pad;
+};
+
+// Batched step (step_tile.hip): one launch advances every lattice of the
+// work table by one step. work[i] = instance << 20 | tile. grid <= nwork
+// blocks; each block walks items blockIdx.x, +gridDim.x, ...
+hipError_t launch_step_batch(int q, int bx, int by, int bz, int storage, const InstArgs* tab, const uint32_t* work,
+                             int nwork, int grid, int k, hipStream_t s);
+hipError_t launch_batch_advance(InstArgs* tab, int ninst, int nsteps, hipStream_t s);
+
 // Launchers (step_tile.hip). Return hipError_t of the launch.
 hipError_t launch_step_tile(int q, int bx, int by, int bz, bool has_bc, int storage, const StepArgs& a,
                             int ntiles, hipStream_t s);

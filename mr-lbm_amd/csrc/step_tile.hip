@@ -84,22 +84,30 @@ __device__ __forceinline__ void halo_store(const StepArgs& a, int lane, int tile
     }
 }
 
+// LDS of one tile update: population arrays with halo, node classes.
+template <int Q, int BX, int BY, int BZ>
+struct TileLds {
+    float s[(Q - 1) * HaloGeom<BX, BY, BZ>::E];
+    uint8_t cls[BX * BY * BZ];
+};
+
+// One time step of one tile (the body of k_step_tile and of the batched
+// kernel). The whole workgroup calls it with the same tile.
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
-__global__ __launch_bounds__(BX* BY* BZ) void k_step_tile(StepArgs a) {
+__device__ __forceinline__ void tile_step(const StepArgs& a, int tile, TileLds<Q, BX, BY, BZ>& L) {
     constexpr int T = BX * BY * BZ;
     constexpr int NW = T / 64;
     using G = HaloGeom<BX, BY, BZ>;
     constexpr int E = G::E;
     constexpr int NIN = n_incoming_chunks<Q, BX, BY, BZ>();
     constexpr int NOUT = n_outgoing_chunks<Q, BX, BY, BZ>();
-    __shared__ float s[(Q - 1) * E];
-    __shared__ uint8_t s_cls[HAS_BC ? T : 1];
+    float* s = L.s;
+    uint8_t* s_cls = L.cls;
 
     const int t = threadIdx.x;
     const int tx = t % BX, ty = (t / BX) % BY, tz = t / (BX * BY);
     const int lane = t % 64;
     const int w = __builtin_amdgcn_readfirstlane(t / 64);
-    const int tile = blockIdx.x;
     const int bx = tile % a.ntx, by = (tile / a.ntx) % a.nty, bz = tile / (a.ntx * a.nty);
     const int base = tx + G::DX * (ty + G::DY * tz);
     // periodic tile neighbours, as the reference's bxm1/bxp1 (src/mlbm.cu:115-122)
@@ -178,6 +186,50 @@ __global__ __launch_bounds__(BX* BY* BZ) void k_step_tile(StepArgs a) {
             });
         }
     });
+}
+
+template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
+__global__ __launch_bounds__(BX* BY* BZ) void k_step_tile(StepArgs a) {
+    __shared__ TileLds<Q, BX, BY, BZ> L;
+    tile_step<Q, BX, BY, BZ, HAS_BC, S>(a, blockIdx.x, L);
+}
+
+template <int Q, int BX, int BY, int BZ, int S>
+__global__ __launch_bounds__(BX* BY* BZ) void k_step_batch(const InstArgs* __restrict__ tab,
+                                                           const uint32_t* __restrict__ work, int nwork, int k) {
+    __shared__ TileLds<Q, BX, BY, BZ> L;
+    for (int item = blockIdx.x; item < nwork; item += gridDim.x) {
+        const uint32_t wv = work[item];
+        const int inst = (int)(wv >> 20), tile = (int)(wv & 0xFFFFFu);
+        const InstArgs& ia = tab[inst];
+        StepArgs a = ia.a;
+        const int p = (ia.cur0 + k) & 1;
+#pragma unroll
+        for (int f = 0; f < 6; f++) {
+            a.gin[f] = ia.g[p][f];
+            a.gout[f] = ia.g[1 - p][f];
+        }
+        a.step = ia.step0 + (uint32_t)k;
+        if (ia.has_bc)
+            tile_step<Q, BX, BY, BZ, true, S>(a, tile, L);
+        else
+            tile_step<Q, BX, BY, BZ, false, S>(a, tile, L);
+        __syncthreads();  // LDS is reused by the next item
+    }
+}
+
+__global__ void k_batch_advance(InstArgs* tab, int ninst, int nsteps) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ninst) {
+        tab[i].step0 += (uint32_t)nsteps;
+        tab[i].cur0 = (tab[i].cur0 + nsteps) & 1;
+    }
+}
+
+hipError_t launch_batch_advance(InstArgs* tab, int ninst, int nsteps, hipStream_t s) {
+    if (ninst <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_advance, dim3((ninst + 255) / 256), dim3(256), 0, s, tab, ninst, nsteps);
+    return hipGetLastError();
 }
 
 // Initial ghost planes: every tile-face node writes its outgoing
@@ -267,6 +319,30 @@ hipError_t launch_init_ghosts(int q, int bx, int by, int bz, int storage, const 
         if (q == 19) launch_ig<19, A, B, C>(storage, a, go, ntiles, s); \
         else launch_ig<27, A, B, C>(storage, a, go, ntiles, s);         \
         return hipGetLastError();                                    \
+    }
+    MRLBM_TILE_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+template <int Q, int BX, int BY, int BZ>
+static hipError_t launch_batch_q(int storage, const InstArgs* tab, const uint32_t* work, int nwork, int grid, int k,
+                                 hipStream_t s) {
+    dim3 g(grid), b(BX * BY * BZ);
+    if (storage == 0) hipLaunchKernelGGL((k_step_batch<Q, BX, BY, BZ, 0>), g, b, 0, s, tab, work, nwork, k);
+    else if (storage == 1) hipLaunchKernelGGL((k_step_batch<Q, BX, BY, BZ, 1>), g, b, 0, s, tab, work, nwork, k);
+    else if (storage == 2) hipLaunchKernelGGL((k_step_batch<Q, BX, BY, BZ, 2>), g, b, 0, s, tab, work, nwork, k);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_step_batch(int q, int bx, int by, int bz, int storage, const InstArgs* tab, const uint32_t* work,
+                             int nwork, int grid, int k, hipStream_t s) {
+    if (nwork <= 0) return hipSuccess;
+#define X(A, B, C)                                                                           \
+    if (bx == A && by == B && bz == C) {                                                     \
+        if (q == 19) return launch_batch_q<19, A, B, C>(storage, tab, work, nwork, grid, k, s); \
+        if (q == 27) return launch_batch_q<27, A, B, C>(storage, tab, work, nwork, grid, k, s); \
     }
     MRLBM_TILE_SHAPES(X)
 #undef X

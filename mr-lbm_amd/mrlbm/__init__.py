@@ -184,3 +184,70 @@ def bc_solve_host(q, node_type, omega, u, pop, rho0=1.0):
     y = yrest + dy
     drho = rho0 * (wk * yrest[0] - 1.0) + r * yrest[0] + rI * dy[0]
     return np.concatenate([[rho0 + drho], np.asarray(u, float), y[1:] / y[0]])
+
+
+class Batch:
+    """Lattices of one type (q, tile, storage) advanced together, one launch
+    per step. Members are `Lattice` objects created with the same q, tile,
+    storage and the tile kernel."""
+
+    def __init__(self, q=19, tile=(8, 8, 8), storage=STORE_FP32, device=0):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(L.mrlbm_batch_create(device, q, tile[0], tile[1], tile[2], storage, ctypes.byref(h)))
+        self._h = h
+        self.members = []
+
+    def add(self, lat):
+        _lib.check(_lib.load().mrlbm_batch_add(self._h, lat._h))
+        self.members.append(lat)
+
+    def remove(self, lat):
+        _lib.check(_lib.load().mrlbm_batch_remove(self._h, lat._h))
+        self.members.remove(lat)
+
+    def set_grid(self, max_blocks):
+        _lib.check(_lib.load().mrlbm_batch_set_grid(self._h, max_blocks))
+
+    def step(self, n, graph_steps=0):
+        _lib.check(_lib.load().mrlbm_batch_step(self._h, n, graph_steps))
+
+    def time_steps(self, n, graph_steps=0):
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().mrlbm_batch_time_steps(self._h, n, graph_steps, ctypes.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        _lib.check(_lib.load().mrlbm_batch_synchronize(self._h))
+
+    def size(self):
+        m, n = ctypes.c_int(), ctypes.c_int64()
+        _lib.check(_lib.load().mrlbm_batch_size(self._h, ctypes.byref(m), ctypes.byref(n)))
+        return m.value, n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().mrlbm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def autotune(dims, shapes=((8, 8, 8), (16, 8, 4), (32, 4, 4), (64, 4, 2)), q=19, storage=STORE_FP32, steps=20,
+             device=0):
+    """Fastest tile shape for batching lattices of extents `dims`; returns
+    (shape, {shape: ms or None})."""
+    sh = (ctypes.c_int * (3 * len(shapes)))(*[v for s in shapes for v in s])
+    dm = (ctypes.c_int * (3 * len(dims)))(*[v for d in dims for v in d])
+    best = ctypes.c_int()
+    ms = np.zeros(len(shapes))
+    _lib.check(_lib.load().mrlbm_batch_autotune(device, q, storage, len(shapes), sh, len(dims), dm, steps,
+                                                ctypes.byref(best), ms.ctypes.data_as(_lib._dp)))
+    return tuple(shapes[best.value]), {tuple(s): (None if m < 0 else float(m)) for s, m in zip(shapes, ms)}

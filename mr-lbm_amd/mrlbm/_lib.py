@@ -68,6 +68,17 @@ _SIGS = {
     "mrlbm_last_error": [],
     "mrlbm_bc_class_info": [ctypes.c_int, ctypes.c_uint32, ctypes.c_double, _dp, _u32p, _dp],
     "mrlbm_copy_bench": [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _dp],
+    "mrlbm_batch_create": [ctypes.c_int] * 6 + [ctypes.POINTER(_vp)],
+    "mrlbm_batch_add": [_vp, _vp],
+    "mrlbm_batch_remove": [_vp, _vp],
+    "mrlbm_batch_set_grid": [_vp, ctypes.c_int],
+    "mrlbm_batch_step": [_vp, ctypes.c_int, ctypes.c_int],
+    "mrlbm_batch_time_steps": [_vp, ctypes.c_int, ctypes.c_int, _dp],
+    "mrlbm_batch_synchronize": [_vp],
+    "mrlbm_batch_size": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64)],
+    "mrlbm_batch_destroy": [_vp],
+    "mrlbm_batch_autotune": [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                             ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp],
 }
 _RESTYPES = {"mrlbm_last_error": ctypes.c_char_p}
 
