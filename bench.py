@@ -59,6 +59,125 @@ def cpu_baseline(target_s):
     }
 
 
+class SplitMix64:
+    """Seeded generator for the synthetic workloads (own implementation so the
+    draws do not depend on a library's distribution code)."""
+
+    def __init__(self, seed):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self, a, b):
+        return a + (b - a) * (self.next() >> 11) / float(1 << 53)
+
+    def choice(self, seq):
+        return seq[self.next() % len(seq)]
+
+
+def batching_cavities(n_lat=64, steps=2000, seed=2024, check=4):
+    """64 lid-driven cavities (edges from {32,48,64,96,128}, Re ~ U[50,400],
+    lid 0.05, tau = 0.5 + 3 u N / Re) in one batch with hipGraph-captured
+    steps and an autotuned tile, against the same lattices stepped one after
+    another with one launch per step."""
+    import numpy as np
+    import mrlbm
+    from mrlbm import geometry
+
+    rng = SplitMix64(seed)
+    lid = 0.05
+    spec = []
+    for _ in range(n_lat):
+        n = rng.choice([32, 48, 64, 96, 128])
+        re = rng.uniform(50.0, 400.0)
+        spec.append((n, 0.5 + 3.0 * lid * n / re))
+    dims = [(n, n, n) for n, _ in spec]
+    shapes = ((8, 8, 8), (16, 8, 4), (32, 4, 4), (64, 4, 2))
+    tile, tune_ms = mrlbm.autotune(dims, shapes=shapes, steps=20)
+    types = {n: geometry.cavity(n) for n in set(n for n, _ in spec)}
+
+    def make(n, tau):
+        lat = mrlbm.Lattice((n, n, n), q=19, tau=tau, walls=(1, 1, 1), tile=tile, kernel=mrlbm.KERNEL_TILE)
+        lat.set_bc_velocity(0, (lid, 0, 0))
+        lat.set_node_types(types[n])
+        lat.init_uniform()
+        return lat
+
+    nodes = sum(n ** 3 for n, _ in spec)
+    gsteps = 50 if steps % 50 == 0 else 1
+    batch = mrlbm.Batch(q=19, tile=tile)
+    lats = [make(n, tau) for n, tau in spec]
+    for lat in lats:
+        batch.add(lat)
+    batch.step(gsteps, graph_steps=gsteps)  # capture + warm-up
+    t0 = time.perf_counter()
+    ms_b = batch.time_steps(steps, graph_steps=gsteps)
+    batch.synchronize()
+    wall_b = time.perf_counter() - t0
+    batched = [lats[i].moments() for i in range(check)]
+    batch.close()
+    for lat in lats:
+        lat.close()
+
+    ms_seq = 0.0
+    identical = True
+    t0 = time.perf_counter()
+    for i, (n, tau) in enumerate(spec):
+        lat = make(n, tau)
+        lat.step(gsteps)
+        ms_seq += lat.time_steps(steps)
+        if i < check:
+            identical &= bool(np.array_equal(lat.moments(), batched[i]))
+        lat.close()
+    wall_seq = time.perf_counter() - t0
+    return {
+        "workload": f"{n_lat} D3Q19 lid-driven cavities, edges from {{32,48,64,96,128}}, Re~U[50,400], "
+                    f"lid 0.05, {steps} steps each (+{gsteps} warm-up), seed {seed}",
+        "total_nodes": nodes,
+        "tile": list(tile),
+        "autotune_ms_20_steps": {"x".join(map(str, k)): v for k, v in tune_ms.items()},
+        "batched_mlups": round(nodes * steps / (ms_b * 1e-3) / 1e6, 1),
+        "sequential_mlups": round(nodes * steps / (ms_seq * 1e-3) / 1e6, 1),
+        "speedup": round(ms_seq / ms_b, 3),
+        "batched_ms": round(ms_b, 2),
+        "sequential_ms": round(ms_seq, 2),
+        "graph_steps": gsteps,
+        "bit_identical_checked": check,
+        "bit_identical": identical,
+    }
+
+
+def serving_ducts(n_req=100, rate=20.0, seed=5):
+    """The duct request stream (Poisson rate 20/s, FP16 storage, 200 GB budget)
+    served in real time with continuous batching; a bounded prefix of the
+    stream (n_req requests) keeps the run to seconds."""
+    from mrlbm import serve
+
+    reqs = serve.duct_requests(n_req, rate=rate, seed=seed)
+    offered = sum(r.nx * r.ny * r.nz * r.steps for r in reqs) / reqs[-1].arrival_s / 1e6
+    st = serve.run(reqs, storage=1, budget_gb=200.0, quantum=100, graph_steps=50, realtime=True)
+    drift = max(abs(r.mass1 - r.mass0) for r in reqs)
+    return {
+        "workload": f"{n_req} duct requests (first {n_req} of the seeded stream, seed {seed}), Poisson {rate}/s, "
+                    "FP16 moments, 200 GB budget, rounds of 100 steps (graphs of 50)",
+        "p50_latency_s": round(st["p50_s"], 3),
+        "p99_latency_s": round(st["p99_s"], 3),
+        "max_latency_s": round(st["max_s"], 3),
+        "aggregate_mlups": round(st["mlups"], 1),
+        "busy_mlups": round(st["mlups_busy"], 1),
+        "utilisation": round(st["busy_s"] / st["wall_s"], 3),
+        "offered_mlups": round(offered, 1),
+        "wall_s": round(st["wall_s"], 2),
+        "peak_state_gb": round(st["peak_bytes"] / 1e9, 2),
+        "max_abs_mean_mass_drift": drift,
+    }
+
+
 def reduced_precision_variants(n, tau, u0, steps, warmup):
     """Same workload with FP16 and BF16 moment storage (FP32 arithmetic)."""
     import mrlbm
@@ -99,6 +218,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip the FP16/BF16 storage lines")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-cavity batching comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served duct stream")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -177,6 +298,10 @@ def main():
             "traffic": traffic,
         },
     }
+    if not args.no_batch:
+        out["batching"] = batching_cavities()
+    if not args.no_serve:
+        out["serving"] = serving_ducts()
     if not args.no_variants:
         out["variants"] = reduced_precision_variants(n, tau, u0, args.steps // 2, args.warmup)
     if not args.no_cpu_baseline:

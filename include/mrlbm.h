@@ -193,6 +193,76 @@ int mrlbm_batch_destroy(mrlbm_batch* b);
 int mrlbm_batch_autotune(int device, int q, int storage, int nshapes, const int* shapes, int ninst, const int* dims,
                          int steps, int* best, double* ms_out);
 
+/* ---------------------------------------------------------------------
+ * Geometry (host): node-type maps in the reference encoding.
+ * mrlbm_classify: domain walls on the wall axes, SOLID_NODE for solid
+ *   voxels, and for fluid nodes next to solid voxels the octants of every
+ *   solid neighbour direction (bc_id, src/lbmInitialization.cu:1080-1227;
+ *   walls and voxels combine by OR; no wrap across a wall axis).
+ *   solid may be NULL. Arrays are x-fastest.
+ * mrlbm_set_moving_wall: non-solid nodes of the domain face `side` (-1/+1)
+ *   of `axis` get BC_VELOCITY_WALL with velocity-table index vel_index.
+ * mrlbm_voxelize_spheres: solid[] = 1 inside any sphere (cx,cy,cz,r) x nsph.
+ * mrlbm_read_voxel_csv: the reference's voxel file (read_xyz_file,
+ *   src/lbmInitialization.cu:978-1039): "x,y,z" per line; returns the
+ *   number of voxels set or a negative status. */
+int mrlbm_classify(int nx, int ny, int nz, int wall_x, int wall_y, int wall_z, const uint8_t* solid,
+                   uint32_t* types);
+int mrlbm_set_moving_wall(int nx, int ny, int nz, int axis, int side, int vel_index, uint32_t* types);
+int mrlbm_voxelize_spheres(int nx, int ny, int nz, int nsph, const double* spheres, uint8_t* solid);
+int mrlbm_read_voxel_csv(const char* path, int nx, int ny, int nz, uint8_t* solid);
+
+/* ---------------------------------------------------------------------
+ * Serving (no reference counterpart): a request loop over independent duct
+ * solves (x periodic, walls on y and z, voxelised spheres, body force gx,
+ * viscosity nu, step budget) with continuous batching. Requests become
+ * visible at arrival_s (realtime = 1) or immediately (realtime = 0), are
+ * admitted when the HBM budget and max_batch allow, keep their fields
+ * resident, advance together one launch per step in rounds of at most
+ * quantum_steps (graph-captured when graph_steps divides a full round) and
+ * leave exactly at their step budget. */
+typedef struct {
+    double arrival_s;           /* in: arrival time, seconds from start         */
+    int nx, ny, nz;             /* in: lattice (multiples of the tile)          */
+    int nspheres;               /* in: 0..8                                     */
+    double sphere[8][4];        /* in: centre x, y, z and radius                */
+    double gx;                  /* in: body force along x                       */
+    double nu;                  /* in: viscosity, tau = 0.5 + 3 nu              */
+    int steps;                  /* in: step budget                              */
+    unsigned int seed;          /* in: stochastic-rounding key                  */
+    double* out_moments;        /* in: optional 10*N buffer for the final state */
+    double start_s, finish_s;   /* out: admission / completion times            */
+    double latency_s;           /* out: completion - arrival                    */
+    double mass0, mass1;        /* out: FP64 mean(rho - 1) before / after       */
+    double ke0, ke1;            /* out: FP64 mean kinetic energy before / after */
+    int status;                 /* out: 0 done                                  */
+} mrlbm_request;
+
+typedef struct {
+    int device, q, storage;
+    int bx, by, bz;             /* 0 = 8x8x8                                    */
+    double hbm_budget_bytes;
+    int quantum_steps;          /* 0 = 100                                      */
+    int graph_steps;            /* 0 = plain launches                           */
+    int max_batch;              /* 0 = no limit (4095)                          */
+    int realtime;
+} mrlbm_serve_params;
+
+typedef struct {
+    double wall_s, busy_s;      /* loop wall time, time inside batch rounds     */
+    double lattice_updates;
+    double mlups, mlups_busy;   /* aggregate over wall / busy time              */
+    double p50_s, p99_s, max_s; /* latency percentiles                          */
+    double peak_bytes;          /* peak admitted state                          */
+    int64_t rounds;
+} mrlbm_serve_stats;
+
+/* The seeded duct request stream (Poisson arrivals at `rate`/s; Nx in
+ * [64,256], Ny = Nz in [32,128], multiples of 16; 1..8 spheres of radius
+ * U[3, Ny/6]; gx = 10^U[-7,-5]; nu = U[0.01,0.1]; steps = U{500..5000}). */
+int mrlbm_generate_duct_requests(uint64_t seed, int n, double rate, mrlbm_request* out);
+int mrlbm_serve_run(const mrlbm_serve_params* params, mrlbm_request* reqs, int nreq, mrlbm_serve_stats* stats);
+
 /* Measurement utility (no reference counterpart): time `iters` plain
  * device-to-device copies of `bytes` on `device`; *ms_per_copy = mean event
  * time of one copy (it reads and writes `bytes` each). */

@@ -143,7 +143,7 @@ __global__ __launch_bounds__(BX* BY) void k_step_column(StepArgs a) {
         collide(m, a.R);
         const int z = bz * BZ + k;
         typename Codec<S>::T qv[10];
-        quantize<S>(m, a, (uint32_t)x + (uint32_t)a.nx * ((uint32_t)y + (uint32_t)a.ny * (uint32_t)z), qv);
+        quantize<S>(m, a, a.step, (uint32_t)x + (uint32_t)a.nx * ((uint32_t)y + (uint32_t)a.ny * (uint32_t)z), qv);
         float g[Q];
         reconstruct<Q>(m, g);
 

@@ -34,14 +34,14 @@ struct Nbr {
 };
 
 template <int Q, int BX, int BY, int BZ, int K>
-__device__ __forceinline__ void halo_load(const StepArgs& a, int lane, const Nbr& nb, float* s) {
+__device__ __forceinline__ void halo_load(const float* const* gin, int lane, const Nbr& nb, float* s) {
     constexpr auto CH = incoming_chunks<Q, BX, BY, BZ>();
     constexpr Chunk c = CH[K];
     constexpr int QF = VS<Q>::QF;
     constexpr int Bu = c.face < 2 ? BY : BX;
     constexpr int Bv = c.face < 4 ? BZ : BY;
     const int otile = nb.tile(c.dx, c.dy, c.dz);
-    const float* __restrict__ src = a.gin[c.face] + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)otile);
+    const float* __restrict__ src = gin[c.face] + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)otile);
     if (lane < c.nu * c.nv) {
         const int lu = lane % c.nu, lv = lane / c.nu;
         const float v = src[(uint32_t)((c.u0 + lu) + Bu * (c.v0 + lv))];
@@ -50,8 +50,8 @@ __device__ __forceinline__ void halo_load(const StepArgs& a, int lane, const Nbr
 }
 
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int K>
-__device__ __forceinline__ void halo_store(const StepArgs& a, int lane, int tile, int bx, int by, int bz,
-                                           const float* s, const uint8_t* s_cls) {
+__device__ __forceinline__ void halo_store(const StepArgs& a, float* const* gout, int lane, int tile, int bx, int by,
+                                           int bz, const float* s, const uint8_t* s_cls) {
     constexpr auto CH = outgoing_chunks<Q, BX, BY, BZ>();
     constexpr Chunk c = CH[K];
     constexpr int QF = VS<Q>::QF;
@@ -67,7 +67,7 @@ __device__ __forceinline__ void halo_store(const StepArgs& a, int lane, int tile
     if constexpr (c.face == 4) skip = a.wall_z && bz == 0;
     if constexpr (c.face == 5) skip = a.wall_z && bz == a.ntz - 1;
     if (skip) return;
-    float* __restrict__ dst = a.gout[c.face] + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)tile);
+    float* __restrict__ dst = gout[c.face] + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)tile);
     if (lane < c.nu * c.nv) {
         const int lu = lane % c.nu, lv = lane / c.nu;
         const int u = c.u0 + lu, v = c.v0 + lv;
@@ -94,7 +94,8 @@ struct TileLds {
 // One time step of one tile (the body of k_step_tile and of the batched
 // kernel). The whole workgroup calls it with the same tile.
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
-__device__ __forceinline__ void tile_step(const StepArgs& a, int tile, TileLds<Q, BX, BY, BZ>& L) {
+__device__ __forceinline__ void tile_step(const StepArgs& a, const float* const* gin, float* const* gout, uint32_t step,
+                                          int tile, TileLds<Q, BX, BY, BZ>& L) {
     constexpr int T = BX * BY * BZ;
     constexpr int NW = T / 64;
     using G = HaloGeom<BX, BY, BZ>;
@@ -120,7 +121,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, int tile, TileLds<Q
 
     uint32_t cls = 0;
     if constexpr (HAS_BC) {
-        cls = a.cls[(size_t)tile * T + t];
+        if (a.cls) cls = a.cls[(size_t)tile * T + t];  // uniform: batched lattices may have no class map
         s_cls[t] = (uint8_t)cls;
     }
     const bool solid = HAS_BC && (cls == 255u);
@@ -135,7 +136,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, int tile, TileLds<Q
         if (w == W) {
             static_for<(NIN + NW - 1) / NW>([&](auto RR) {
                 constexpr int K = W + NW * decltype(RR)::value;
-                if constexpr (K < NIN) halo_load<Q, BX, BY, BZ, K>(a, lane, nb, s);
+                if constexpr (K < NIN) halo_load<Q, BX, BY, BZ, K>(gin, lane, nb, s);
             });
         }
     });
@@ -164,7 +165,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, int tile, TileLds<Q
     collide(m, a.R);
     typename Codec<S>::T q[10];
     const uint32_t node_id = (uint32_t)(bx * BX + tx) + (uint32_t)a.nx * ((uint32_t)(by * BY + ty) + (uint32_t)a.ny * (uint32_t)(bz * BZ + tz));
-    quantize<S>(m, a, node_id, q);
+    quantize<S>(m, a, step, node_id, q);
     reconstruct<Q>(m, g);
     if (!solid) store_node<Q, BX, BY, BZ, S>(a, mb, q);
 
@@ -182,7 +183,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, int tile, TileLds<Q
         if (w == W) {
             static_for<(NOUT + NW - 1) / NW>([&](auto RR) {
                 constexpr int K = W + NW * decltype(RR)::value;
-                if constexpr (K < NOUT) halo_store<Q, BX, BY, BZ, HAS_BC, K>(a, lane, tile, bx, by, bz, s, s_cls);
+                if constexpr (K < NOUT) halo_store<Q, BX, BY, BZ, HAS_BC, K>(a, gout, lane, tile, bx, by, bz, s, s_cls);
             });
         }
     });
@@ -191,7 +192,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, int tile, TileLds<Q
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
 __global__ __launch_bounds__(BX* BY* BZ) void k_step_tile(StepArgs a) {
     __shared__ TileLds<Q, BX, BY, BZ> L;
-    tile_step<Q, BX, BY, BZ, HAS_BC, S>(a, blockIdx.x, L);
+    tile_step<Q, BX, BY, BZ, HAS_BC, S>(a, a.gin, a.gout, a.step, blockIdx.x, L);
 }
 
 template <int Q, int BX, int BY, int BZ, int S>
@@ -199,21 +200,13 @@ __global__ __launch_bounds__(BX* BY* BZ) void k_step_batch(const InstArgs* __res
                                                            const uint32_t* __restrict__ work, int nwork, int k) {
     __shared__ TileLds<Q, BX, BY, BZ> L;
     for (int item = blockIdx.x; item < nwork; item += gridDim.x) {
-        const uint32_t wv = work[item];
+        const uint32_t wv = __builtin_amdgcn_readfirstlane(work[item]);
         const int inst = (int)(wv >> 20), tile = (int)(wv & 0xFFFFFu);
-        const InstArgs& ia = tab[inst];
-        StepArgs a = ia.a;
+        const InstArgs& ia = tab[inst];  // read in place (scalar loads)
         const int p = (ia.cur0 + k) & 1;
-#pragma unroll
-        for (int f = 0; f < 6; f++) {
-            a.gin[f] = ia.g[p][f];
-            a.gout[f] = ia.g[1 - p][f];
-        }
-        a.step = ia.step0 + (uint32_t)k;
-        if (ia.has_bc)
-            tile_step<Q, BX, BY, BZ, true, S>(a, tile, L);
-        else
-            tile_step<Q, BX, BY, BZ, false, S>(a, tile, L);
+        // one body for every member (a lattice without a class map reads
+        // class 0): one register budget, the occupancy of the plain kernel
+        tile_step<Q, BX, BY, BZ, true, S>(ia.a, ia.g[p], ia.g[1 - p], ia.step0 + (uint32_t)k, tile, L);
         __syncthreads();  // LDS is reused by the next item
     }
 }

@@ -56,7 +56,8 @@ __device__ __forceinline__ uint32_t xs32(uint32_t x) {
 }
 
 template <int S>
-__device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t node_id, typename Codec<S>::T* q) {
+__device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t step, uint32_t node_id,
+                                         typename Codec<S>::T* q) {
     using C = Codec<S>;
     if constexpr (S == 0) {
         q[0] = n.drho;
@@ -70,7 +71,7 @@ __device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t no
         q[8] = n.myz;
         q[9] = n.mzz;
     } else {
-        const uint32_t w0 = sr_bits(node_id, a.step, a.seed);
+        const uint32_t w0 = sr_bits(node_id, step, a.seed);
         const uint32_t w1 = xs32(w0), w2 = xs32(w1);
         constexpr uint32_t SH = S == 1 ? 5u : 8u;            // random byte -> dropped-bit position
         constexpr uint32_t HALF = S == 1 ? 16u : 128u;       // centre of the byte's interval

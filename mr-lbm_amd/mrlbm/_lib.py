@@ -35,6 +35,35 @@ class Info(ctypes.Structure):
     ]
 
 
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("arrival_s", ctypes.c_double), ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("nspheres", ctypes.c_int), ("sphere", (ctypes.c_double * 4) * 8), ("gx", ctypes.c_double),
+        ("nu", ctypes.c_double), ("steps", ctypes.c_int), ("seed", ctypes.c_uint),
+        ("out_moments", ctypes.POINTER(ctypes.c_double)), ("start_s", ctypes.c_double),
+        ("finish_s", ctypes.c_double), ("latency_s", ctypes.c_double), ("mass0", ctypes.c_double),
+        ("mass1", ctypes.c_double), ("ke0", ctypes.c_double), ("ke1", ctypes.c_double), ("status", ctypes.c_int),
+    ]
+
+
+class ServeParams(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int), ("q", ctypes.c_int), ("storage", ctypes.c_int), ("bx", ctypes.c_int),
+        ("by", ctypes.c_int), ("bz", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_double),
+        ("quantum_steps", ctypes.c_int), ("graph_steps", ctypes.c_int), ("max_batch", ctypes.c_int),
+        ("realtime", ctypes.c_int),
+    ]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("wall_s", ctypes.c_double), ("busy_s", ctypes.c_double), ("lattice_updates", ctypes.c_double),
+        ("mlups", ctypes.c_double), ("mlups_busy", ctypes.c_double), ("p50_s", ctypes.c_double),
+        ("p99_s", ctypes.c_double), ("max_s", ctypes.c_double), ("peak_bytes", ctypes.c_double),
+        ("rounds", ctypes.c_int64),
+    ]
+
+
 class MrlbmError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"mrlbm error {code}: {msg}")
@@ -79,6 +108,13 @@ _SIGS = {
     "mrlbm_batch_destroy": [_vp],
     "mrlbm_batch_autotune": [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                              ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp],
+    "mrlbm_classify": [ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_uint8), _u32p],
+    "mrlbm_set_moving_wall": [ctypes.c_int] * 6 + [_u32p],
+    "mrlbm_voxelize_spheres": [ctypes.c_int] * 4 + [_dp, ctypes.POINTER(ctypes.c_uint8)],
+    "mrlbm_read_voxel_csv": [ctypes.c_char_p] + [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_uint8)],
+    "mrlbm_generate_duct_requests": [ctypes.c_uint64, ctypes.c_int, ctypes.c_double, ctypes.POINTER(Request)],
+    "mrlbm_serve_run": [ctypes.POINTER(ServeParams), ctypes.POINTER(Request), ctypes.c_int,
+                        ctypes.POINTER(ServeStats)],
 }
 _RESTYPES = {"mrlbm_last_error": ctypes.c_char_p}
 

@@ -32,67 +32,38 @@ def octants_of(d):
     return m
 
 
-def wall_types(shape, walls):
-    """Octant bits of the domain walls on the wall axes."""
-    nx, ny, nz = shape
-    t = np.zeros(shape, np.uint32)
-    n = (nx, ny, nz)
-    for a in range(3):
-        if not walls[a]:
-            continue
-        lo = [slice(None)] * 3
-        hi = [slice(None)] * 3
-        lo[a] = 0
-        hi[a] = n[a] - 1
-        dlo = [0, 0, 0]
-        dhi = [0, 0, 0]
-        dlo[a] = -1
-        dhi[a] = 1
-        t[tuple(lo)] |= octants_of(dlo)
-        t[tuple(hi)] |= octants_of(dhi)
-    return t
-
-
 def classify(shape, walls=(0, 0, 0), solid=None, moving=None):
-    """Node-type map.
+    """Node-type map (native, mrlbm_classify / mrlbm_set_moving_wall).
 
     solid: bool array (nx, ny, nz) of solid voxels, or None.
     moving: list of (axis, side, vel_index): every non-solid node of the
       domain face `side` (-1 low, +1 high) of `axis` becomes a fixed-velocity
       wall node using UX_BC/UY_BC/UZ_BC[vel_index] (e.g. a cavity lid).
     """
-    t = wall_types(shape, walls)
+    from . import _lib
+    import ctypes
+    L = _lib.load()
+    nx, ny, nz = shape
+    t = np.zeros(nx * ny * nz, np.uint32)
+    sp = None
     if solid is not None:
-        solid = np.asarray(solid, bool)
-        acc = np.zeros(shape, np.uint32)
-        for d in itertools.product((-1, 0, 1), repeat=3):
-            if d == (0, 0, 0):
-                continue
-            # neighbour at x + d is solid -> octants containing d
-            nb = solid
-            valid = np.ones(shape, bool)
-            for a in range(3):
-                if d[a] == 0:
-                    continue
-                nb = np.roll(nb, -d[a], axis=a)
-                if walls[a]:
-                    # no wrap across a wall axis
-                    idx = [slice(None)] * 3
-                    idx[a] = -1 if d[a] > 0 else 0
-                    valid[tuple(idx)] = False
-            acc |= np.where(nb & valid, np.uint32(octants_of(d)), np.uint32(0))
-        t |= acc
-        t[solid] = SOLID_NODE
-    if moving:
-        n = shape
-        for axis, side, k in moving:
-            sl = [slice(None)] * 3
-            sl[axis] = 0 if side < 0 else n[axis] - 1
-            face = t[tuple(sl)]
-            fluid = (face & 0xFF) != SOLID_NODE
-            face[fluid] |= np.uint32(BC_VELOCITY_WALL | (k << VEL_SHIFT))
-            t[tuple(sl)] = face
-    return t
+        s = np.ascontiguousarray(np.asarray(solid, bool).reshape(-1, order="F"), dtype=np.uint8)
+        sp = s.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    _lib.check(L.mrlbm_classify(nx, ny, nz, walls[0], walls[1], walls[2], sp, t.ctypes.data_as(_lib._u32p)))
+    for axis, side, k in moving or []:
+        _lib.check(L.mrlbm_set_moving_wall(nx, ny, nz, axis, side, k, t.ctypes.data_as(_lib._u32p)))
+    return t.reshape(shape, order="F")
+
+
+def read_voxel_csv(path, shape):
+    """Solid mask from the reference's voxel CSV ("x,y,z" per line)."""
+    from . import _lib
+    import ctypes
+    s = np.zeros(int(np.prod(shape)), np.uint8)
+    n = _lib.load().mrlbm_read_voxel_csv(str(path).encode(), *shape, s.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    if n < 0:
+        _lib.check(n)
+    return s.reshape(shape, order="F").astype(bool)
 
 
 def cavity(n, lid_axis=2):
@@ -103,9 +74,11 @@ def cavity(n, lid_axis=2):
 
 
 def spheres(shape, centres, radii):
-    """Boolean voxel mask of spheres (node centres inside radius)."""
-    x, y, z = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
-    m = np.zeros(shape, bool)
-    for (cx, cy, cz), r in zip(centres, radii):
-        m |= (x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2 <= r * r
-    return m
+    """Boolean voxel mask of spheres (node centres inside radius; native)."""
+    from . import _lib
+    import ctypes
+    sph = np.ascontiguousarray([[c[0], c[1], c[2], r] for c, r in zip(centres, radii)], dtype=np.float64).reshape(-1)
+    s = np.zeros(int(np.prod(shape)), np.uint8)
+    _lib.check(_lib.load().mrlbm_voxelize_spheres(*shape, len(radii), sph.ctypes.data_as(_lib._dp),
+                                                  s.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+    return s.reshape(shape, order="F").astype(bool)

@@ -127,3 +127,32 @@ def test_invalid_parameters_rejected():
         mrlbm.Lattice((8, 8, 8), tau=0.5)
     with pytest.raises(mrlbm.MrlbmError):
         mrlbm.Lattice((12, 8, 8), tile=(8, 8, 8))
+
+
+def test_duct_request_stream():
+    from mrlbm import serve
+    a = serve.duct_requests(400, rate=20.0, seed=11)
+    b = serve.duct_requests(400, rate=20.0, seed=11)
+    arr = np.array([r.arrival_s for r in a])
+    assert np.all(np.diff(arr) > 0)
+    assert abs(400 / arr[-1] - 20.0) < 3.0  # Poisson rate
+    for r, s in zip(a, b):
+        assert (r.nx, r.ny, r.nz, r.steps, r.nspheres, r.seed) == (s.nx, s.ny, s.nz, s.steps, s.nspheres, s.seed)
+        assert r.nx % 16 == 0 and 64 <= r.nx <= 256 and r.ny == r.nz and r.ny % 16 == 0 and 32 <= r.ny <= 128
+        assert 1 <= r.nspheres <= 8 and 500 <= r.steps <= 5000
+        assert 1e-7 <= r.gx <= 1e-5 and 0.01 <= r.nu <= 0.1
+        for k in range(r.nspheres):
+            assert 3.0 <= r.sphere[k][3] <= max(3.0, r.ny / 6.0)
+
+
+def test_read_voxel_csv(tmp_path):
+    p = tmp_path / "v.csv"
+    p.write_text("1,2,3\n0,0,0\n9,9,9\n4,1,2\n")
+    s = geometry.read_voxel_csv(p, (5, 4, 4))
+    assert s.sum() == 3 and s[1, 2, 3] and s[0, 0, 0] and s[4, 1, 2]
+
+
+def test_spheres_voxelisation():
+    s = geometry.spheres((16, 16, 16), [(8, 8, 8)], [3.0])
+    x, y, z = np.meshgrid(*[np.arange(16)] * 3, indexing="ij")
+    np.testing.assert_array_equal(s, (x - 8) ** 2 + (y - 8) ** 2 + (z - 8) ** 2 <= 9)
