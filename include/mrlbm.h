@@ -99,6 +99,18 @@ int mrlbm_set_bc_velocity(mrlbm_ctx* ctx, int k, double ux, double uy, double uz
  * NULL = every node BULK (fully periodic lattice). */
 int mrlbm_set_node_types(mrlbm_ctx* ctx, const uint32_t* node_types);
 
+/* No-slip geometry classified on the device: the lattice's wall axes
+ * (params wall_x/y/z) plus solid voxels, from a host mask (x-fastest, N
+ * bytes, nonzero = solid; NULL for none) and/or spheres (nspheres records
+ * cx, cy, cz, r; a node is solid when its squared distance to a centre is
+ * at most r^2). Node types and boundary rules are those that
+ * mrlbm_voxelize_spheres + mrlbm_classify + mrlbm_set_node_types give, so
+ * the stepped fields are identical, without host passes over the lattice.
+ * Replaces the host triple loops of hostInitialization_nodeType and
+ * define_voxel_bc (src/lbmInitialization.cu:958-975, :1107-1227) for
+ * wall and voxel cases without moving walls. */
+int mrlbm_set_geometry(mrlbm_ctx* ctx, const uint8_t* solid, int nspheres, const double* spheres);
+
 /* Initial macroscopic fields (x-fastest, unscaled): moments and both ghost
  * sets. Replaces gpuInitialization_mom + gpuInitialization_pop
  * (src/lbmInitialization.cu:28-218, :220-335). Resets the step counter. */

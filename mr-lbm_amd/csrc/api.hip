@@ -273,19 +273,29 @@ void node_moments(int q, int mode, double rho0, double rho, double ux, double uy
     m[9] = 4.5 * pzz;
 }
 
+// One stored moment (index m, stored convention) -> its storage word
+uint32_t encode_value(const mrlbm_ctx* c, int m, double x) {
+    const double v = x * c->mscale[m];
+    if (c->p.storage == MRLBM_STORE_FP32) {
+        const float f = (float)v;
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return u;
+    }
+    return c->p.storage == MRLBM_STORE_FP16 ? to_half(v) : to_bf16(v);
+}
+
 // Moments (block layout, doubles, stored convention) -> storage bytes
 std::vector<char> encode_block(const mrlbm_ctx* c, const std::vector<double>& blk) {
     const size_t sb = storage_bytes(c->p.storage);
     std::vector<char> raw(blk.size() * sb);
     const size_t T = c->T;
     for (size_t j = 0; j < blk.size(); j++) {
-        const int m = (int)((j / T) % 10);
-        const double v = blk[j] * c->mscale[m];
-        if (c->p.storage == MRLBM_STORE_FP32) {
-            const float f = (float)v;
-            std::memcpy(&raw[j * 4], &f, 4);
+        const uint32_t u = encode_value(c, (int)((j / T) % 10), blk[j]);
+        if (sb == 4) {
+            std::memcpy(&raw[j * 4], &u, 4);
         } else {
-            const uint16_t h = c->p.storage == MRLBM_STORE_FP16 ? to_half(v) : to_bf16(v);
+            const uint16_t h = (uint16_t)u;
             std::memcpy(&raw[j * 2], &h, 2);
         }
     }
@@ -313,11 +323,8 @@ std::vector<double> decode_block(const mrlbm_ctx* c, const std::vector<char>& ra
     return blk;
 }
 
-int upload_and_ghosts(mrlbm_ctx* c, const std::vector<double>& blk) {
-    HIPCHK(hipSetDevice(c->p.device));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    const std::vector<char> raw = encode_block(c, blk);
-    HIPCHK(hipMemcpy(c->d_mom, raw.data(), raw.size(), hipMemcpyHostToDevice));
+// Both ghost sets from the stored moments; the lattice restarts at step 0.
+int ghosts_from_moments(mrlbm_ctx* c) {
     StepArgs a = c->args();
     for (int s = 0; s < 2; s++) {
         if (c->p.kernel == MRLBM_KERNEL_COLUMN)
@@ -333,6 +340,14 @@ int upload_and_ghosts(mrlbm_ctx* c, const std::vector<double>& blk) {
     c->initialised = true;
     c->version++;
     return MRLBM_OK;
+}
+
+int upload_and_ghosts(mrlbm_ctx* c, const std::vector<double>& blk) {
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const std::vector<char> raw = encode_block(c, blk);
+    HIPCHK(hipMemcpy(c->d_mom, raw.data(), raw.size(), hipMemcpyHostToDevice));
+    return ghosts_from_moments(c);
 }
 
 template <typename F>
@@ -372,12 +387,18 @@ int mrlbm_init_fields(mrlbm_ctx* c, const double* rho, const double* ux, const d
 
 int mrlbm_init_uniform(mrlbm_ctx* c, double rho, double ux, double uy, double uz, int mode) {
     if (!c) return fail(MRLBM_EINVAL, "null context");
-    return init_with(c, mode, [&](int, int, int, double& r, double& a, double& b, double& d) {
-        r = rho;
-        a = ux;
-        b = uy;
-        d = uz;
-    });
+    if (mode != MRLBM_INIT_EQUILIBRIUM && mode != MRLBM_INIT_REFERENCE)
+        return fail(MRLBM_EINVAL, "unknown second-moment mode %d", mode);
+    if (!(rho > 0)) return fail(MRLBM_EINVAL, "non-positive density");
+    // one node's stored moments, encoded once and filled on the device
+    double m[10];
+    node_moments(c->p.q, mode, c->p.rho0, rho, ux, uy, uz, m);
+    uint32_t enc[10];
+    for (int k = 0; k < 10; k++) enc[k] = encode_value(c, k, m[k]);
+    HIPCHK(hipSetDevice(c->p.device));
+    int rc = fill_uniform_moments(c, enc);
+    if (rc) return rc;
+    return ghosts_from_moments(c);
 }
 
 int mrlbm_init_taylor_green(mrlbm_ctx* c, double u0, int mode) {

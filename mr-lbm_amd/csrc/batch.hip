@@ -152,6 +152,8 @@ int do_batch_steps(mrlbm_batch* b, int nsteps, int use_graph) {
 
 }  // namespace
 
+hipStream_t mrlbm::batch_stream(const mrlbm_batch* b) { return b->stream; }
+
 extern "C" {
 
 int mrlbm_batch_create(int device, int q, int bx, int by, int bz, int storage, mrlbm_batch** out) {

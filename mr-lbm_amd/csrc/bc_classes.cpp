@@ -114,6 +114,40 @@ bool build_bc_inverse(int q, uint32_t known, double omega, const double u[3], do
     return true;
 }
 
+bool class_record(int q, uint32_t octants, double omega, const double u[3], float* rec) {
+    const uint32_t known = known_mask(q, octants);
+    double ainv[49];
+    if (!build_bc_inverse(q, known, omega, u, ainv)) return false;
+    std::memset(rec, 0, sizeof(float) * BC_REC);
+    std::memcpy(&rec[0], &known, 4);
+    rec[1] = (float)u[0];
+    rec[2] = (float)u[1];
+    rec[3] = (float)u[2];
+    for (int k = 0; k < 49; k++) rec[4 + k] = (float)ainv[k];
+    // rest-state split of the right-hand side (populations travel
+    // as f_i - w_i RHO_0): W_K, c = sum_K w_i H_i / W_K, and the rest
+    // solution y_rest = A^-1 (1, c), d0 = W_K y_rest,0 - 1
+    double wk = 0, wh[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < q; i++) {
+        if (!((known >> i) & 1u)) continue;
+        double h[6];
+        hermite(i, h);
+        wk += weight(q, i);
+        for (int j = 0; j < 6; j++) wh[j] += weight(q, i) * h[j];
+    }
+    double brest[7] = {1.0, wh[0] / wk, wh[1] / wk, wh[2] / wk, wh[3] / wk, wh[4] / wk, wh[5] / wk};
+    double yrest[7];
+    for (int k = 0; k < 7; k++) {
+        yrest[k] = 0;
+        for (int j = 0; j < 7; j++) yrest[k] += ainv[7 * k + j] * brest[j];
+    }
+    rec[53] = (float)wk;
+    for (int j = 0; j < 6; j++) rec[54 + j] = (float)brest[1 + j];
+    rec[60] = (float)(wk * yrest[0] - 1.0);
+    for (int k = 0; k < 7; k++) rec[61 + k] = (float)yrest[k];
+    return true;
+}
+
 std::string build_class_table(int q, double omega, const double bc_vel[4][3], const uint32_t* types, size_t n,
                               BcClassTable& out) {
     out = BcClassTable();
@@ -148,40 +182,12 @@ std::string build_class_table(int q, double omega, const double bc_vel[4][3], co
                        " (only no-slip and fixed-velocity walls)";
             }
             if ((int)ids.size() >= 254) return "more than 254 distinct boundary classes";
-            const uint32_t known = known_mask(q, oct);
-            double ainv[49];
-            if (!build_bc_inverse(q, known, omega, u, ainv))
+            float rec[BC_REC];
+            if (!class_record(q, oct, omega, u, rec))
                 return "singular boundary system for node type " + std::to_string(key);
             id = (int)ids.size() + 1;
             ids[key] = id;
             out.keys.push_back(key);
-            float rec[BC_REC] = {};
-            std::memcpy(&rec[0], &known, 4);
-            rec[1] = (float)u[0];
-            rec[2] = (float)u[1];
-            rec[3] = (float)u[2];
-            for (int k = 0; k < 49; k++) rec[4 + k] = (float)ainv[k];
-            // rest-state split of the right-hand side (populations travel
-            // as f_i - w_i RHO_0): W_K, c = sum_K w_i H_i / W_K, and the rest
-            // solution y_rest = A^-1 (1, c), d0 = W_K y_rest,0 - 1
-            double wk = 0, wh[6] = {0, 0, 0, 0, 0, 0};
-            for (int i = 0; i < q; i++) {
-                if (!((known >> i) & 1u)) continue;
-                double h[6];
-                hermite(i, h);
-                wk += weight(q, i);
-                for (int j = 0; j < 6; j++) wh[j] += weight(q, i) * h[j];
-            }
-            double brest[7] = {1.0, wh[0] / wk, wh[1] / wk, wh[2] / wk, wh[3] / wk, wh[4] / wk, wh[5] / wk};
-            double yrest[7];
-            for (int k = 0; k < 7; k++) {
-                yrest[k] = 0;
-                for (int j = 0; j < 7; j++) yrest[k] += ainv[7 * k + j] * brest[j];
-            }
-            rec[53] = (float)wk;
-            for (int j = 0; j < 6; j++) rec[54 + j] = (float)brest[1 + j];
-            rec[60] = (float)(wk * yrest[0] - 1.0);
-            for (int k = 0; k < 7; k++) rec[61 + k] = (float)yrest[k];
             out.table.insert(out.table.end(), rec, rec + BC_REC);
         } else {
             id = it->second;
@@ -191,6 +197,14 @@ std::string build_class_table(int q, double omega, const double bc_vel[4][3], co
     }
     if (out.has_solid) out.has_bc = true;
     return "";
+}
+
+void build_octant_table(int q, double omega, std::vector<float>& table, uint32_t singular[8]) {
+    table.assign(256 * BC_REC, 0.0f);
+    for (int w = 0; w < 8; w++) singular[w] = 0;
+    const double u[3] = {0, 0, 0};
+    for (uint32_t oct = 1; oct < 255; oct++)
+        if (!class_record(q, oct, omega, u, &table[oct * BC_REC])) singular[oct >> 5] |= 1u << (oct & 31);
 }
 
 }  // namespace mrlbm

@@ -27,6 +27,16 @@ struct BcClassTable {
     bool has_bc = false, has_solid = false;
 };
 
+// Class record (BC_STRIDE floats, layout of lattice.h) of a boundary node
+// with the given octant byte and wall velocity u. False if singular.
+bool class_record(int q, uint32_t octants, double omega, const double u[3], float* rec);
+
+// Records of every no-slip octant pattern, indexed by the octant byte
+// itself (256 records; 0 bulk and 255 solid unused), for class maps built
+// on the device where class id == octant byte. Patterns whose system is
+// singular are flagged in `singular` (bit oct of the 256-bit set).
+void build_octant_table(int q, double omega, std::vector<float>& table, uint32_t singular[8]);
+
 // Map a reference-encoded node-type map to class ids (0 bulk, 255 solid,
 // 1..254 boundary classes) and build the device table. bc_vel is the
 // UX_BC/UY_BC/UZ_BC table (4 x 3). Returns an empty string on success.

@@ -108,3 +108,12 @@ struct mrlbm_ctx {
     }
 };
 
+namespace mrlbm {
+
+// Every node's ten stored moments set on the device to `enc` (storage words).
+int fill_uniform_moments(mrlbm_ctx* c, const uint32_t enc[10]);
+
+// The stream a batch launches on (members' work is ordered on it).
+hipStream_t batch_stream(const mrlbm_batch* b);
+
+}  // namespace mrlbm

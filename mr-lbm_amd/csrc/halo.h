@@ -151,7 +151,7 @@ constexpr int outgoing_chunks_impl(Chunk* out) {
                     int node[3] = {0, 0, 0};
                     node[a] = (face & 1) ? G::B[a] - 1 : 0;
                     node[va] = v;
-                    int e[3];
+                    int e[3] = {0, 0, 0};
                     for (int b = 0; b < 3; b++) e[b] = node[b] + (c[b] > 0 ? 1 : 0);
                     ch.lds0 = (i - 1) * G::E + e[0] + G::DX * (e[1] + G::DY * e[2]);
                     ch.su = G::STRIDE[ua];

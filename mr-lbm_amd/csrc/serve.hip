@@ -5,14 +5,16 @@
 // One host thread runs the loop. Requests become visible at their arrival
 // time (wall clock, or immediately in as-fast-as-possible mode); a visible
 // request is admitted when the HBM budget and the batch-size limit allow:
-// its lattice is allocated, classified (duct walls on y and z, voxelised
-// spheres) and initialised at rest, and it joins the batch. All active
-// lattices advance together, one launch per step, in rounds of at most
-// `quantum` steps (hipGraph-captured when a full quantum fits); a round
-// never passes a member's step budget, so members leave exactly at their
-// budget. A finished request is reduced (FP64 mass), optionally read back,
-// and its memory is released before the next admission check. Latency is
-// arrival to results-on-host.
+// its lattice is allocated, classified on the device (duct walls on y and
+// z, voxelised spheres) and initialised at rest on its own stream while the
+// current round runs, and it joins the batch at the next round boundary.
+// All active lattices advance together, one launch per step, in rounds of
+// at most `quantum` steps (hipGraph-captured when a full quantum fits); a
+// round never passes a member's step budget, so members leave exactly at
+// their budget. A finished request leaves the batch at the round boundary;
+// its FP64 mass (and optionally its moments) are read during the next round
+// and its memory is released at the boundary after that. Latency is arrival
+// to results-on-host.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,10 +27,6 @@
 #include "context.h"
 
 using namespace mrlbm;
-
-extern "C" int mrlbm_classify(int nx, int ny, int nz, int wall_x, int wall_y, int wall_z, const uint8_t* solid,
-                              uint32_t* types);
-extern "C" int mrlbm_voxelize_spheres(int nx, int ny, int nz, int nsph, const double* sph, uint8_t* solid);
 
 namespace {
 
@@ -103,6 +101,8 @@ int mrlbm_serve_run(const mrlbm_serve_params* sp, mrlbm_request* reqs, int nreq,
     for (int i = 0; i < nreq; i++) {
         if (reqs[i].nx % bx || reqs[i].ny % by || reqs[i].nz % bz)
             return fail(MRLBM_EINVAL, "request %d extents are not multiples of the tile", i);
+        if (reqs[i].nspheres < 0 || reqs[i].nspheres > 8)
+            return fail(MRLBM_EINVAL, "request %d has %d spheres (0..8)", i, reqs[i].nspheres);
         if (state_bytes(reqs[i], sp->q, sp->storage, bx, by, bz) > sp->hbm_budget_bytes)
             return fail(MRLBM_ENOMEM, "request %d alone exceeds the HBM budget", i);
         if (i && reqs[i].arrival_s < reqs[i - 1].arrival_s)
@@ -112,26 +112,40 @@ int mrlbm_serve_run(const mrlbm_serve_params* sp, mrlbm_request* reqs, int nreq,
     mrlbm_batch* batch = nullptr;
     int rc = mrlbm_batch_create(sp->device, sp->q, bx, by, bz, sp->storage, &batch);
     if (rc) return rc;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess) {
+        mrlbm_batch_destroy(batch);
+        return fail(MRLBM_EHIP, "cannot create events");
+    }
 
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
-    std::vector<Active> active;
+    // ready: admitted and initialised, joins at the next round boundary;
+    // active: batch members; leaving: out of the batch, results being read
+    std::vector<Active> ready, active, leaving;
     double used = 0, peak = 0, busy = 0, lu = 0;
-    int next = 0, done = 0;
-    std::vector<uint8_t> solid;
-    std::vector<uint32_t> types;
+    int next = 0, done = 0, round_q = 0;
+    bool in_flight = false;
 
-    auto fail_all = [&](int code) {
-        for (auto& a : active) mrlbm_destroy(a.ctx);
+    auto release = [&]() {
+        for (auto* v : {&ready, &active, &leaving})
+            for (auto& a : *v) mrlbm_destroy(a.ctx);
         mrlbm_batch_destroy(batch);
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+    };
+    auto fail_all = [&](int code) {
+        release();
         return code;
     };
-
-    while (done < nreq) {
-        // admissions
+    // Admit visible requests while the budget and the batch limit allow;
+    // runs while a round is in flight (its work goes to each lattice's own
+    // stream), so set-up overlaps stepping.
+    auto admit = [&]() -> int {
         const double tnow = now();
-        while (next < nreq && (!sp->realtime || reqs[next].arrival_s <= tnow) && (int)active.size() < max_batch) {
+        while (next < nreq && (!sp->realtime || reqs[next].arrival_s <= tnow) &&
+               (int)(active.size() + ready.size()) < max_batch) {
             mrlbm_request& r = reqs[next];
             const double bytes = state_bytes(r, sp->q, sp->storage, bx, by, bz);
             if (used + bytes > sp->hbm_budget_bytes) break;
@@ -152,64 +166,93 @@ int mrlbm_serve_run(const mrlbm_serve_params* sp, mrlbm_request* reqs, int nreq,
             p.device = sp->device;
             p.seed = r.seed;
             mrlbm_ctx* c = nullptr;
-            if ((rc = mrlbm_create(&p, &c))) return fail_all(rc);
-            const size_t n = (size_t)r.nx * r.ny * r.nz;
-            solid.assign(n, 0);
-            types.assign(n, 0);
-            mrlbm_voxelize_spheres(r.nx, r.ny, r.nz, r.nspheres, &r.sphere[0][0], solid.data());
-            mrlbm_classify(r.nx, r.ny, r.nz, 0, 1, 1, solid.data(), types.data());
-            if ((rc = mrlbm_set_node_types(c, types.data())) ||
-                (rc = mrlbm_init_uniform(c, 1.0, 0.0, 0.0, 0.0, MRLBM_INIT_EQUILIBRIUM)) ||
-                (rc = mrlbm_reduce(c, &r.mass0, &r.ke0)) || (rc = mrlbm_batch_add(batch, c))) {
+            int e = mrlbm_create(&p, &c);
+            if (!e) e = mrlbm_set_geometry(c, nullptr, r.nspheres, &r.sphere[0][0]);
+            if (!e) e = mrlbm_init_uniform(c, 1.0, 0.0, 0.0, 0.0, MRLBM_INIT_EQUILIBRIUM);
+            if (!e) e = mrlbm_reduce(c, &r.mass0, &r.ke0);
+            if (e) {
                 mrlbm_destroy(c);
-                return fail_all(rc);
+                return e;
             }
             r.start_s = now();
             r.status = 2;  // running
-            active.push_back({next, c, r.steps, bytes});
+            ready.push_back({next, c, r.steps, bytes});
             used += bytes;
             peak = std::max(peak, used);
             next++;
         }
+        return MRLBM_OK;
+    };
+    // Results of lattices that left the batch: FP64 mass, optional moments.
+    auto finish = [&]() -> int {
+        for (auto& a : leaving) {
+            mrlbm_request& r = reqs[a.idx];
+            int e = mrlbm_reduce(a.ctx, &r.mass1, &r.ke1);
+            if (!e && r.out_moments) e = mrlbm_read_moments(a.ctx, r.out_moments);
+            if (e) return e;
+            r.finish_s = now();
+            r.latency_s = r.finish_s - (sp->realtime ? r.arrival_s : 0.0);
+            r.status = 0;
+            done++;
+        }
+        return MRLBM_OK;
+    };
+
+    while (done < nreq) {
+        if ((rc = admit())) return fail_all(rc);
+        if ((rc = finish())) return fail_all(rc);
+        if (in_flight) {
+            // end of the round in flight
+            if (hipEventSynchronize(ev1) != hipSuccess) return fail_all(fail(MRLBM_EHIP, "round failed"));
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, ev0, ev1);
+            busy += ms * 1e-3;
+            st->rounds++;
+            in_flight = false;
+            for (auto& a : active) {
+                a.remaining -= round_q;
+                lu += (double)round_q * a.ctx->N;
+            }
+        }
+        // memory of finished requests is released while the GPU is idle
+        for (auto& a : leaving) {
+            mrlbm_destroy(a.ctx);
+            used -= a.bytes;
+        }
+        leaving.clear();
+        for (size_t i = 0; i < active.size();) {
+            if (active[i].remaining > 0) {
+                i++;
+                continue;
+            }
+            if ((rc = mrlbm_batch_remove(batch, active[i].ctx))) return fail_all(rc);
+            leaving.push_back(active[i]);
+            active.erase(active.begin() + i);
+        }
+        for (auto& a : ready) {
+            if ((rc = mrlbm_batch_add(batch, a.ctx))) return fail_all(rc);
+            active.push_back(a);
+        }
+        ready.clear();
         if (active.empty()) {
-            if (next < nreq && sp->realtime) {
+            if (!leaving.empty()) continue;  // read their results now
+            if ((rc = admit())) return fail_all(rc);
+            if (ready.empty() && next < nreq && sp->realtime) {
                 const double wait = reqs[next].arrival_s - now();
                 if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
             }
             continue;
         }
-        // one round for every active lattice
+        // next round: at most `quantum` steps, never past a member's budget
         int q = quantum;
         for (auto& a : active) q = std::min(q, a.remaining);
-        const auto r0 = clk::now();
-        rc = mrlbm_batch_step(batch, q, (graph && q == quantum) ? graph : 0);
-        if (!rc) rc = mrlbm_batch_synchronize(batch);
-        if (rc) return fail_all(rc);
-        busy += std::chrono::duration<double>(clk::now() - r0).count();
-        st->rounds++;
-        // completions
-        for (size_t i = 0; i < active.size();) {
-            Active& a = active[i];
-            a.remaining -= q;
-            lu += (double)q * a.ctx->N;
-            if (a.remaining > 0) {
-                i++;
-                continue;
-            }
-            mrlbm_request& r = reqs[a.idx];
-            if ((rc = mrlbm_reduce(a.ctx, &r.mass1, &r.ke1))) return fail_all(rc);
-            if (r.out_moments && (rc = mrlbm_read_moments(a.ctx, r.out_moments))) return fail_all(rc);
-            r.finish_s = now();
-            r.latency_s = r.finish_s - (sp->realtime ? r.arrival_s : 0.0);
-            r.status = 0;
-            mrlbm_batch_remove(batch, a.ctx);
-            mrlbm_destroy(a.ctx);
-            used -= a.bytes;
-            done++;
-            active.erase(active.begin() + i);
-        }
+        if (hipEventRecord(ev0, batch_stream(batch)) != hipSuccess) return fail_all(fail(MRLBM_EHIP, "event record"));
+        if ((rc = mrlbm_batch_step(batch, q, (graph && q == quantum) ? graph : 0))) return fail_all(rc);
+        if (hipEventRecord(ev1, batch_stream(batch)) != hipSuccess) return fail_all(fail(MRLBM_EHIP, "event record"));
+        round_q = q;
+        in_flight = true;
     }
-    mrlbm_batch_destroy(batch);
+    release();
     const double wall = now();
     std::vector<double> lat;
     for (int i = 0; i < nreq; i++) lat.push_back(reqs[i].latency_s);

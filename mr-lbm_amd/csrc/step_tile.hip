@@ -33,8 +33,12 @@ struct Nbr {
     }
 };
 
+// Ghost value of incoming chunk K for this lane. Every lane loads (lanes
+// past the chunk re-read its first cell, a line the chunk reads anyway), so
+// a wave's loads of all its chunks issue back to back with no exec-mask
+// branch between them and one wait covers them all.
 template <int Q, int BX, int BY, int BZ, int K>
-__device__ __forceinline__ void halo_load(const float* const* gin, int lane, const Nbr& nb, float* s) {
+__device__ __forceinline__ float halo_fetch(const float* const* gin, int lane, const Nbr& nb) {
     constexpr auto CH = incoming_chunks<Q, BX, BY, BZ>();
     constexpr Chunk c = CH[K];
     constexpr int QF = VS<Q>::QF;
@@ -42,9 +46,17 @@ __device__ __forceinline__ void halo_load(const float* const* gin, int lane, con
     constexpr int Bv = c.face < 4 ? BZ : BY;
     const int otile = nb.tile(c.dx, c.dy, c.dz);
     const float* __restrict__ src = gin[c.face] + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)otile);
+    const int lu = lane % c.nu, lv = lane / c.nu;
+    const uint32_t idx = lane < c.nu * c.nv ? (uint32_t)((c.u0 + lu) + Bu * (c.v0 + lv)) : (uint32_t)(c.u0 + Bu * c.v0);
+    return src[idx];
+}
+
+template <int Q, int BX, int BY, int BZ, int K>
+__device__ __forceinline__ void halo_put(int lane, float v, float* s) {
+    constexpr auto CH = incoming_chunks<Q, BX, BY, BZ>();
+    constexpr Chunk c = CH[K];
     if (lane < c.nu * c.nv) {
         const int lu = lane % c.nu, lv = lane / c.nu;
-        const float v = src[(uint32_t)((c.u0 + lu) + Bu * (c.v0 + lv))];
         s[c.lds0 + lu * c.su + lv * c.sv] = v;
     }
 }
@@ -130,13 +142,20 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const float* const*
     Node n;
     load_node<Q, BX, BY, BZ, S>(a, mb, n);
 
-    // populations entering the tile: neighbour ghost planes -> LDS halo
+    // populations entering the tile: neighbour ghost planes -> LDS halo;
+    // wave W owns chunks W, W + NW, ...: all loads first, then the writes
     static_for<NW>([&](auto WW) {
         constexpr int W = decltype(WW)::value;
+        constexpr int R = (NIN - W + NW - 1) / NW;
         if (w == W) {
-            static_for<(NIN + NW - 1) / NW>([&](auto RR) {
-                constexpr int K = W + NW * decltype(RR)::value;
-                if constexpr (K < NIN) halo_load<Q, BX, BY, BZ, K>(gin, lane, nb, s);
+            float hv[R > 0 ? R : 1];
+            static_for<R>([&](auto RR) {
+                constexpr int r = decltype(RR)::value;
+                hv[r] = halo_fetch<Q, BX, BY, BZ, W + NW * r>(gin, lane, nb);
+            });
+            static_for<R>([&](auto RR) {
+                constexpr int r = decltype(RR)::value;
+                halo_put<Q, BX, BY, BZ, W + NW * r>(lane, hv[r], s);
             });
         }
     });

@@ -78,6 +78,17 @@ class Lattice:
         t = _flat(types, self.n, np.uint32)
         _lib.check(L.mrlbm_set_node_types(self._h, t.ctypes.data_as(_lib._u32p)))
 
+    def set_geometry(self, solid=None, centres=(), radii=()):
+        """No-slip geometry classified on the device: the lattice's walls, a
+        solid mask (flat x-fastest, or an (nx, ny, nz) array) and spheres."""
+        L = _lib.load()
+        sp = np.ascontiguousarray([[*c, r] for c, r in zip(centres, radii)], dtype=np.float64).reshape(-1, 4)
+        m = None
+        if solid is not None:
+            m = _flat(solid, self.n, np.uint8)
+        _lib.check(L.mrlbm_set_geometry(self._h, None if m is None else m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                        len(sp), sp.ctypes.data_as(_lib._dp) if len(sp) else None))
+
     def init_fields(self, rho, ux, uy, uz, mode=INIT_EQUILIBRIUM):
         a = [_flat(v, self.n, np.float64) for v in (rho, ux, uy, uz)]
         _lib.check(_lib.load().mrlbm_init_fields(self._h, *[v.ctypes.data_as(_lib._dp) for v in a], mode))

@@ -81,6 +81,7 @@ _SIGS = {
     "mrlbm_destroy": [_vp],
     "mrlbm_set_bc_velocity": [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double],
     "mrlbm_set_node_types": [_vp, _u32p],
+    "mrlbm_set_geometry": [_vp, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int, _dp],
     "mrlbm_init_fields": [_vp, _dp, _dp, _dp, _dp, ctypes.c_int],
     "mrlbm_init_uniform": [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int],
     "mrlbm_init_taylor_green": [_vp, ctypes.c_double, ctypes.c_int],
