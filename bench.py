@@ -298,13 +298,21 @@ def main():
             "traffic": traffic,
         },
     }
+    def phase(name):
+        print(f"bench: {name}", file=sys.stderr, flush=True)
+
+    phase(f"headline {mlups:.1f} MLUPS")
     if not args.no_batch:
+        phase("64-cavity batch")
         out["batching"] = batching_cavities()
     if not args.no_serve:
+        phase("served duct stream")
         out["serving"] = serving_ducts()
     if not args.no_variants:
+        phase("FP16/BF16 storage")
         out["variants"] = reduced_precision_variants(n, tau, u0, args.steps // 2, args.warmup)
     if not args.no_cpu_baseline:
+        phase("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)
 

@@ -58,7 +58,7 @@ typedef struct mrlbm_ctx mrlbm_ctx;
 typedef struct {
     int nx, ny, nz;        /* NX, NY, NZ                                        */
     int q;                 /* 19 (D3Q19) or 27 (D3Q27)                          */
-    int bx, by, bz;        /* tile shape; 0,0,0 = chosen by the library         */
+    int bx, by, bz;        /* tile shape; 0,0,0 = library default (8x8x8 tiles) */
     int storage;           /* MRLBM_STORE_*                                     */
     int wall_x, wall_y, wall_z; /* 0 periodic (BC_*_PERIODIC), 1 wall (BC_*_WALL) */
     int kernel;            /* MRLBM_KERNEL_*                                    */

@@ -89,16 +89,13 @@ int mrlbm_create(const mrlbm_params* pp, mrlbm_ctx** out) {
     int bx = p.bx, by = p.by, bz = p.bz;
     int kernel = p.kernel;
     if (bx == 0 && by == 0 && bz == 0) {
-        // default: the column kernel with the longest segment that divides
-        // the lattice, else 8x8x8 tiles
-        const int cand[3][3] = {{16, 16, 64}, {16, 16, 32}, {8, 8, 8}};
+        // default: 8x8x8 tiles (the fastest measured shape), or for the
+        // column kernel the longest segment that divides the lattice
+        const int cand[3][3] = {{8, 8, 8}, {16, 16, 64}, {16, 16, 32}};
         for (int c = 0; c < 3; c++) {
             const bool col = cand[c][0] * cand[c][1] == 256;
-            if (kernel == MRLBM_KERNEL_TILE && col) continue;
+            if (kernel != MRLBM_KERNEL_COLUMN && col) continue;
             if (kernel == MRLBM_KERNEL_COLUMN && !col) continue;
-            // the column kernel is the default only where it is built and
-            // fastest: D3Q19 with FP32 storage
-            if (kernel == MRLBM_KERNEL_AUTO && col && (p.q != 19 || p.storage != MRLBM_STORE_FP32)) continue;
             if (p.nx % cand[c][0] == 0 && p.ny % cand[c][1] == 0 && p.nz % cand[c][2] == 0) {
                 bx = cand[c][0];
                 by = cand[c][1];
@@ -211,6 +208,7 @@ int mrlbm_set_node_types(mrlbm_ctx* c, const uint32_t* types) {
     c->has_solid = tab.has_solid;
     c->n_classes = (int)tab.keys.size() - 1;
     c->version++;
+    c->cls_version++;
     if (!c->has_bc) return MRLBM_OK;
     std::vector<uint8_t> blk(c->N);
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;

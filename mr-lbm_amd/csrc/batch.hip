@@ -26,9 +26,10 @@ struct mrlbm_batch {
     std::vector<uint64_t> seen;  // member state versions the device table reflects
     InstArgs* d_tab = nullptr;
     int tab_cap = 0;
-    uint32_t* d_work = nullptr;
+    uint32_t* d_work = nullptr;  // bulk-tile items, then boundary-tile items
     int work_cap = 0;
     int nwork = 0;
+    int nbulk = 0;
     bool dirty = true;
     int grid_limit = 0;
     hipGraph_t graph = nullptr;
@@ -61,7 +62,7 @@ int refresh(mrlbm_batch* b) {
     const int n = (int)b->members.size();
     if (n > 4095) return fail(MRLBM_EINVAL, "a batch holds at most 4095 lattices");
     std::vector<InstArgs> tab(n);
-    std::vector<uint32_t> work;
+    std::vector<uint32_t> work, bc_work;
     for (int i = 0; i < n; i++) {
         mrlbm_ctx* c = b->members[i];
         if (!c->initialised) return fail(MRLBM_ESTATE, "batch member %d is not initialised", i);
@@ -74,8 +75,14 @@ int refresh(mrlbm_batch* b) {
         ia.step0 = (uint32_t)c->step;
         ia.has_bc = c->has_bc ? 1 : 0;
         tab[i] = ia;
-        for (int t = 0; t < c->ntiles; t++) work.push_back(((uint32_t)i << 20) | (uint32_t)t);
+        const std::vector<uint8_t>* flags = nullptr;
+        int rc = tile_flags(c, &flags);
+        if (rc) return rc;
+        for (int t = 0; t < c->ntiles; t++)
+            ((*flags)[t] ? bc_work : work).push_back(((uint32_t)i << 20) | (uint32_t)t);
     }
+    const int nbulk = (int)work.size();
+    work.insert(work.end(), bc_work.begin(), bc_work.end());
     HIPCHK(hipSetDevice(b->device));
     HIPCHK(hipStreamSynchronize(b->stream));
     if (n > b->tab_cap) {
@@ -93,6 +100,7 @@ int refresh(mrlbm_batch* b) {
     if (n) HIPCHK(hipMemcpy(b->d_tab, tab.data(), sizeof(InstArgs) * n, hipMemcpyHostToDevice));
     if (!work.empty()) HIPCHK(hipMemcpy(b->d_work, work.data(), sizeof(uint32_t) * work.size(), hipMemcpyHostToDevice));
     b->nwork = (int)work.size();
+    b->nbulk = nbulk;
     for (int i = 0; i < n; i++) b->seen[i] = b->members[i]->version;
     b->dirty = false;
     drop_graph(b);
@@ -100,10 +108,17 @@ int refresh(mrlbm_batch* b) {
 }
 
 int enqueue_steps(mrlbm_batch* b, int nsteps) {
-    const int grid = b->grid_limit > 0 ? std::min(b->grid_limit, b->nwork) : b->nwork;
+    // one launch over the all-bulk tiles, one over the boundary tiles; the
+    // two read the same ghost set and write disjoint data
+    const int nb = b->nbulk, nc = b->nwork - b->nbulk;
+    const int gb = b->grid_limit > 0 ? std::min(b->grid_limit, nb) : nb;
+    const int gc = b->grid_limit > 0 ? std::min(b->grid_limit, nc) : nc;
     for (int k = 0; k < nsteps; k++) {
-        hipError_t e =
-            launch_step_batch(b->q, b->bx, b->by, b->bz, b->storage, b->d_tab, b->d_work, b->nwork, grid, k, b->stream);
+        hipError_t e = launch_step_batch(b->q, b->bx, b->by, b->bz, b->storage, false, b->d_tab, b->d_work, nb, gb, k,
+                                         b->stream);
+        if (e == hipSuccess)
+            e = launch_step_batch(b->q, b->bx, b->by, b->bz, b->storage, true, b->d_tab, b->d_work + nb, nc, gc, k,
+                                  b->stream);
         if (e != hipSuccess) return fail(MRLBM_EHIP, "batched step launch failed: %s", hipGetErrorString(e));
     }
     hipError_t e = launch_batch_advance(b->d_tab, (int)b->members.size(), nsteps, b->stream);

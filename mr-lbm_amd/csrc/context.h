@@ -49,6 +49,9 @@ struct mrlbm_ctx {
     uint64_t version = 0;          // bumped when buffers or state are replaced outside a step
     hipStream_t own_stream = nullptr;
     float mscale[10], mrscale[10];  // storage scales (powers of two)
+    uint64_t cls_version = 0;        // bumped when the class map is replaced
+    std::vector<uint8_t> tile_flags;  // per tile: 1 = holds a boundary or solid node
+    uint64_t tile_flags_version = ~0ull;
 
     size_t mom_index(int x, int y, int z, int m) const {
         const int tile = (x / bx) + ntx * ((y / by) + nty * (z / bz));
@@ -112,6 +115,10 @@ namespace mrlbm {
 
 // Every node's ten stored moments set on the device to `enc` (storage words).
 int fill_uniform_moments(mrlbm_ctx* c, const uint32_t enc[10]);
+
+// Per-tile boundary flags of the current class map (computed on the device
+// once per class map).
+int tile_flags(mrlbm_ctx* c, const std::vector<uint8_t>** out);
 
 // The stream a batch launches on (members' work is ordered on it).
 hipStream_t batch_stream(const mrlbm_batch* b);

@@ -18,8 +18,8 @@ namespace mrlbm {
 template <int S> struct Codec;
 template <> struct Codec<0> {
     using T = float;
-    __device__ __forceinline__ static float ld(const T* p, size_t i, float) { return p[i]; }
-    __device__ __forceinline__ static void st(T* p, size_t i, float v, float) { p[i] = v; }
+    template <typename P>
+    __device__ __forceinline__ static float ld(P p, size_t i, float) { return p[i]; }
     __device__ __forceinline__ static T enc(float v, float, uint32_t) { return v; }
     __device__ __forceinline__ static float dec(T b, float) { return b; }
 };
@@ -33,7 +33,8 @@ template <> struct Codec<1> {  // IEEE binary16, 13 dropped bits rounded stochas
         const float t = __uint_as_float((u + (rnd & 0x1FFFu)) & ~0x1FFFu);
         return __builtin_bit_cast(T, (_Float16)t);  // exact for normal binary16 values
     }
-    __device__ __forceinline__ static float ld(const T* p, size_t i, float rscale) { return dec(p[i], rscale); }
+    template <typename P>
+    __device__ __forceinline__ static float ld(P p, size_t i, float rscale) { return dec(p[i], rscale); }
 };
 template <> struct Codec<2> {  // bfloat16, 16 dropped bits rounded stochastically
     using T = uint16_t;
@@ -42,7 +43,8 @@ template <> struct Codec<2> {  // bfloat16, 16 dropped bits rounded stochastical
         const uint32_t u = __float_as_uint(v * scale);
         return (T)((u + (rnd & 0xFFFFu)) >> 16);
     }
-    __device__ __forceinline__ static float ld(const T* p, size_t i, float rscale) { return dec(p[i], rscale); }
+    template <typename P>
+    __device__ __forceinline__ static float ld(P p, size_t i, float rscale) { return dec(p[i], rscale); }
 };
 
 // Counter-based random bits for stochastic rounding: a 32-bit mix of the
@@ -111,9 +113,10 @@ struct InstArgs {
 
 // Batched step (step_tile.hip): one launch advances every lattice of the
 // work table by one step. work[i] = instance << 20 | tile. grid <= nwork
-// blocks; each block walks items blockIdx.x, +gridDim.x, ...
-hipError_t launch_step_batch(int q, int bx, int by, int bz, int storage, const InstArgs* tab, const uint32_t* work,
-                             int nwork, int grid, int k, hipStream_t s);
+// blocks; block b takes the contiguous items [b*per, (b+1)*per), per =
+// ceil(nwork/grid). has_bc = false only for tiles whose nodes are all bulk.
+hipError_t launch_step_batch(int q, int bx, int by, int bz, int storage, bool has_bc, const InstArgs* tab,
+                             const uint32_t* work, int nwork, int grid, int k, hipStream_t s);
 hipError_t launch_batch_advance(InstArgs* tab, int ninst, int nsteps, hipStream_t s);
 
 // Launchers (step_tile.hip). Return hipError_t of the launch.

@@ -25,6 +25,17 @@
 
 namespace mrlbm {
 
+// Global-address-space view of a device pointer. Pointers that kernels read
+// from device tables are generic to the compiler, which then emits flat_*
+// accesses; those also count against the LDS wait counter, so every LDS
+// wait would stall on outstanding HBM traffic. Accesses through this view
+// are global_* instructions.
+#define MRLBM_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ MRLBM_GLOBAL T* gptr(T* p) {
+    return (MRLBM_GLOBAL T*)p;
+}
+
 // Population ordering of src/includeFiles/velocitySets/D3Q27.inc:44-46;
 // the first 19 entries are the D3Q19 set (D3Q19.inc:17-19).
 __host__ __device__ constexpr int cxv(int i) {
@@ -227,8 +238,8 @@ constexpr int BC_STRIDE = 72;
 // r = sum_K g and T = sum_K g H, so every term the FP32 arithmetic forms is
 // a small deviation. Outputs unscaled rho, u, m exactly as a
 // bc_definition.inc fragment does (src/mlbm.cu:456-461); the caller scales.
-template <int Q>
-__device__ __forceinline__ void bc_solve(const float* __restrict__ c, const float* g, float rho0, Node& n) {
+template <int Q, typename P>
+__device__ __forceinline__ void bc_solve(P c, const float* g, float rho0, Node& n) {
     const uint32_t known = __float_as_uint(c[0]);
     float r = 0.f, sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
 #pragma unroll

@@ -137,6 +137,15 @@ __global__ void k_fill_moments(W* __restrict__ mom, size_t n, int T, W v0, W v1,
     mom[j] = out;
 }
 
+// flags[tile] = 1 if any node of the tile is not bulk (one block per tile)
+__global__ void k_tile_flags(const uint8_t* __restrict__ cls, int T, uint8_t* __restrict__ flags) {
+    const uint8_t* c = cls + (size_t)blockIdx.x * T;
+    int any = 0;
+    for (int t = threadIdx.x; t < T; t += blockDim.x) any |= c[t];
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) flags[blockIdx.x] = any ? 1 : 0;
+}
+
 Extents extents_of(const mrlbm_ctx* c) {
     Extents e{};
     e.nx = c->p.nx;
@@ -174,6 +183,25 @@ int fill_uniform_moments(mrlbm_ctx* c, const uint32_t enc[10]) {
                                                                h[4], h[5], h[6], h[7], h[8], h[9]);
     }
     HIPCHK(hipGetLastError());
+    return MRLBM_OK;
+}
+
+int tile_flags(mrlbm_ctx* c, const std::vector<uint8_t>** out) {
+    if (c->tile_flags_version != c->cls_version || c->tile_flags.size() != (size_t)c->ntiles) {
+        c->tile_flags.assign(c->ntiles, 0);
+        if (c->has_bc && c->d_cls) {
+            HIPCHK(hipSetDevice(c->p.device));
+            uint8_t* d = nullptr;
+            HIPCHK(hipMallocAsync((void**)&d, c->ntiles, c->stream));
+            k_tile_flags<<<c->ntiles, 256, 0, c->stream>>>(c->d_cls, c->T, d);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(c->tile_flags.data(), d, c->ntiles, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipFreeAsync(d, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        c->tile_flags_version = c->cls_version;
+    }
+    *out = &c->tile_flags;
     return MRLBM_OK;
 }
 
@@ -254,6 +282,7 @@ int mrlbm_set_geometry(mrlbm_ctx* c, const uint8_t* solid, int nspheres, const d
     c->has_bc = any;
     c->n_classes = n_classes;
     c->version++;
+    c->cls_version++;
     return MRLBM_OK;
 }
 

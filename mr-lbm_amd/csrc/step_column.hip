@@ -135,7 +135,7 @@ __global__ __launch_bounds__(BX* BY) void k_step_column(StepArgs a) {
 
         Node m;
         if (HAS_BC && cls != 0u && cls != 255u) {
-            bc_solve<Q>(a.bc + (size_t)cls * BC_STRIDE, in0, a.rho0, m);
+            bc_solve<Q>(gptr(a.bc) + (size_t)cls * BC_STRIDE, in0, a.rho0, m);
             scale_moments(m);
         } else {
             bulk_moments<Q>(in0, a.rho0, m);

@@ -45,7 +45,7 @@ __device__ __forceinline__ float halo_fetch(const float* const* gin, int lane, c
     constexpr int Bu = c.face < 2 ? BY : BX;
     constexpr int Bv = c.face < 4 ? BZ : BY;
     const int otile = nb.tile(c.dx, c.dy, c.dz);
-    const float* __restrict__ src = gin[c.face] + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)otile);
+    const auto src = gptr(gin[c.face]) + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)otile);
     const int lu = lane % c.nu, lv = lane / c.nu;
     const uint32_t idx = lane < c.nu * c.nv ? (uint32_t)((c.u0 + lu) + Bu * (c.v0 + lv)) : (uint32_t)(c.u0 + Bu * c.v0);
     return src[idx];
@@ -79,7 +79,7 @@ __device__ __forceinline__ void halo_store(const StepArgs& a, float* const* gout
     if constexpr (c.face == 4) skip = a.wall_z && bz == 0;
     if constexpr (c.face == 5) skip = a.wall_z && bz == a.ntz - 1;
     if (skip) return;
-    float* __restrict__ dst = gout[c.face] + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)tile);
+    const auto dst = gptr(gout[c.face]) + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)tile);
     if (lane < c.nu * c.nv) {
         const int lu = lane % c.nu, lv = lane / c.nu;
         const int u = c.u0 + lu, v = c.v0 + lv;
@@ -133,7 +133,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const float* const*
 
     uint32_t cls = 0;
     if constexpr (HAS_BC) {
-        if (a.cls) cls = a.cls[(size_t)tile * T + t];  // uniform: batched lattices may have no class map
+        if (a.cls) cls = gptr(a.cls)[(size_t)tile * T + t];  // uniform: batched lattices may have no class map
         s_cls[t] = (uint8_t)cls;
     }
     const bool solid = HAS_BC && (cls == 255u);
@@ -176,7 +176,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const float* const*
 
     Node m;
     if (HAS_BC && cls != 0u && cls != 255u) {
-        bc_solve<Q>(a.bc + (size_t)cls * BC_STRIDE, g, a.rho0, m);
+        bc_solve<Q>(gptr(a.bc) + (size_t)cls * BC_STRIDE, g, a.rho0, m);
         scale_moments(m);
     } else {
         bulk_moments<Q>(g, a.rho0, m);
@@ -214,18 +214,24 @@ __global__ __launch_bounds__(BX* BY* BZ) void k_step_tile(StepArgs a) {
     tile_step<Q, BX, BY, BZ, HAS_BC, S>(a, a.gin, a.gout, a.step, blockIdx.x, L);
 }
 
-template <int Q, int BX, int BY, int BZ, int S>
+// Batched step: each block takes a contiguous range of work items (the
+// item's work word and instance arguments then mostly hit the scalar cache).
+// HAS_BC = false runs tiles whose nodes are all bulk (the lighter body, more
+// workgroups per CU); HAS_BC = true runs the tiles that hold boundary or
+// solid nodes, and tiles of lattices without a class map read class 0.
+template <int Q, int BX, int BY, int BZ, int S, bool HAS_BC>
 __global__ __launch_bounds__(BX* BY* BZ) void k_step_batch(const InstArgs* __restrict__ tab,
                                                            const uint32_t* __restrict__ work, int nwork, int k) {
     __shared__ TileLds<Q, BX, BY, BZ> L;
-    for (int item = blockIdx.x; item < nwork; item += gridDim.x) {
+    const int per = (nwork + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int begin = (int)blockIdx.x * per;
+    const int end = begin + per < nwork ? begin + per : nwork;
+    for (int item = begin; item < end; item++) {
         const uint32_t wv = __builtin_amdgcn_readfirstlane(work[item]);
         const int inst = (int)(wv >> 20), tile = (int)(wv & 0xFFFFFu);
         const InstArgs& ia = tab[inst];  // read in place (scalar loads)
         const int p = (ia.cur0 + k) & 1;
-        // one body for every member (a lattice without a class map reads
-        // class 0): one register budget, the occupancy of the plain kernel
-        tile_step<Q, BX, BY, BZ, true, S>(ia.a, ia.g[p], ia.g[1 - p], ia.step0 + (uint32_t)k, tile, L);
+        tile_step<Q, BX, BY, BZ, HAS_BC, S>(ia.a, ia.g[p], ia.g[1 - p], ia.step0 + (uint32_t)k, tile, L);
         __syncthreads();  // LDS is reused by the next item
     }
 }
@@ -337,24 +343,31 @@ hipError_t launch_init_ghosts(int q, int bx, int by, int bz, int storage, const 
     return hipErrorInvalidValue;
 }
 
-template <int Q, int BX, int BY, int BZ>
-static hipError_t launch_batch_q(int storage, const InstArgs* tab, const uint32_t* work, int nwork, int grid, int k,
-                                 hipStream_t s) {
+template <int Q, int BX, int BY, int BZ, bool HAS_BC>
+static hipError_t launch_batch_qb(int storage, const InstArgs* tab, const uint32_t* work, int nwork, int grid, int k,
+                                  hipStream_t s) {
     dim3 g(grid), b(BX * BY * BZ);
-    if (storage == 0) hipLaunchKernelGGL((k_step_batch<Q, BX, BY, BZ, 0>), g, b, 0, s, tab, work, nwork, k);
-    else if (storage == 1) hipLaunchKernelGGL((k_step_batch<Q, BX, BY, BZ, 1>), g, b, 0, s, tab, work, nwork, k);
-    else if (storage == 2) hipLaunchKernelGGL((k_step_batch<Q, BX, BY, BZ, 2>), g, b, 0, s, tab, work, nwork, k);
+    if (storage == 0) hipLaunchKernelGGL((k_step_batch<Q, BX, BY, BZ, 0, HAS_BC>), g, b, 0, s, tab, work, nwork, k);
+    else if (storage == 1) hipLaunchKernelGGL((k_step_batch<Q, BX, BY, BZ, 1, HAS_BC>), g, b, 0, s, tab, work, nwork, k);
+    else if (storage == 2) hipLaunchKernelGGL((k_step_batch<Q, BX, BY, BZ, 2, HAS_BC>), g, b, 0, s, tab, work, nwork, k);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-hipError_t launch_step_batch(int q, int bx, int by, int bz, int storage, const InstArgs* tab, const uint32_t* work,
-                             int nwork, int grid, int k, hipStream_t s) {
+template <int Q, int BX, int BY, int BZ>
+static hipError_t launch_batch_q(int storage, bool has_bc, const InstArgs* tab, const uint32_t* work, int nwork,
+                                 int grid, int k, hipStream_t s) {
+    return has_bc ? launch_batch_qb<Q, BX, BY, BZ, true>(storage, tab, work, nwork, grid, k, s)
+                  : launch_batch_qb<Q, BX, BY, BZ, false>(storage, tab, work, nwork, grid, k, s);
+}
+
+hipError_t launch_step_batch(int q, int bx, int by, int bz, int storage, bool has_bc, const InstArgs* tab,
+                             const uint32_t* work, int nwork, int grid, int k, hipStream_t s) {
     if (nwork <= 0) return hipSuccess;
-#define X(A, B, C)                                                                           \
-    if (bx == A && by == B && bz == C) {                                                     \
-        if (q == 19) return launch_batch_q<19, A, B, C>(storage, tab, work, nwork, grid, k, s); \
-        if (q == 27) return launch_batch_q<27, A, B, C>(storage, tab, work, nwork, grid, k, s); \
+#define X(A, B, C)                                                                                   \
+    if (bx == A && by == B && bz == C) {                                                             \
+        if (q == 19) return launch_batch_q<19, A, B, C>(storage, has_bc, tab, work, nwork, grid, k, s); \
+        if (q == 27) return launch_batch_q<27, A, B, C>(storage, has_bc, tab, work, nwork, grid, k, s); \
     }
     MRLBM_TILE_SHAPES(X)
 #undef X

@@ -28,7 +28,7 @@ template <int Q, int BX, int BY, int BZ, int S>
 __device__ __forceinline__ void load_node(const StepArgs& a, size_t mb, Node& n) {
     using C = Codec<S>;
     constexpr size_t T = (size_t)BX * BY * BZ;
-    const typename C::T* mom = (const typename C::T*)a.mom;
+    const auto mom = gptr((const typename C::T*)a.mom);
     n.drho = C::ld(mom, mb + 0 * T, a.mrscale[0]);
     n.rho = a.rho0 + n.drho;
     n.ux = C::ld(mom, mb + 1 * T, a.mrscale[1]);
@@ -97,7 +97,7 @@ __device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t st
 template <int Q, int BX, int BY, int BZ, int S>
 __device__ __forceinline__ void store_node(const StepArgs& a, size_t mb, const typename Codec<S>::T* q) {
     constexpr size_t T = (size_t)BX * BY * BZ;
-    typename Codec<S>::T* mom = (typename Codec<S>::T*)a.mom;
+    const auto mom = gptr((typename Codec<S>::T*)a.mom);
 #pragma unroll
     for (int k = 0; k < 10; k++) mom[mb + k * T] = q[k];
 }
@@ -126,7 +126,7 @@ template <int Q, int BX, int BY, int BZ, int FACE>
 __device__ __forceinline__ void get_face(const StepArgs& a, int tx, int ty, int tz, int bx, int by, int bz,
                                          float* pop) {
     constexpr int QF = VS<Q>::QF;
-    const float* __restrict__ g = a.gin[FACE];
+    const auto g = gptr(a.gin[FACE]);
     static_for<QF>([&](auto SI) {
         constexpr int s = decltype(SI)::value;
         constexpr int p = face_pop(FACE, s);
