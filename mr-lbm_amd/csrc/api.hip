@@ -419,7 +419,7 @@ static int do_steps(mrlbm_ctx* c, int nsteps) {
         hipError_t e =
             c->p.kernel == MRLBM_KERNEL_COLUMN
                 ? launch_step_column(c->p.q, c->bx, c->by, c->bz, c->has_bc, c->p.storage, a, c->ntiles, c->stream)
-                : launch_step_tile(c->p.q, c->bx, c->by, c->bz, c->has_bc, c->p.storage, a, c->ntiles, c->stream);
+                : launch_step_tile(c->p.q, c->bx, c->by, c->bz, c->has_bc, c->p.storage, a, c->ntiles, 0, c->stream);
         if (e != hipSuccess) return fail(MRLBM_EHIP, "step launch failed: %s", hipGetErrorString(e));
         c->cur = 1 - c->cur;
         c->step++;

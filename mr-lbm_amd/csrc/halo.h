@@ -7,8 +7,8 @@
 // base address plus a compile-time offset: no wrap arithmetic, no branch.
 // The extra plane on the upwind side of each array holds the populations
 // that enter the tile; they are copied from the neighbour tiles' ghost
-// planes by whole waves ("chunks"), each chunk a box of at most 64 cells
-// that is contiguous in the ghost plane. When a source node lies outside
+// planes by whole waves ("chunks"), each chunk a window of whole rows of a
+// ghost plane (at most 64 cells, contiguous in HBM). When a source node lies outside
 // the tile along several axes, the value comes from the face of the last of
 // those axes in X, Y, Z order, which is the branch that wins in the
 // reference's popLoad.inc (src/includeFiles/popLoad.inc:2-86).
@@ -20,15 +20,23 @@
 
 namespace mrlbm {
 
+// A chunk is a window of whole rows of one face plane of one slot: `rows`
+// rows of Bu cells (rows * Bu <= 64), lane l <-> face cell (u, v) =
+// (l % Bu, w0 + l / Bu), so its ghost-plane index is l + Bu * w0 (a
+// constant offset from the lane; lanes past the window re-read it). Lanes
+// outside [ulo, uhi] x [vlo, vhi] only load. Bu (the face's first
+// coordinate extent) must divide 64.
 struct Chunk {
     int face;        // ghost face array (0 X_0, 1 X_1, 2 Y_0, 3 Y_1, 4 Z_0, 5 Z_1)
-    int dx, dy, dz;  // offset of the ghost-plane owner tile
+    int dx, dy, dz;  // offset of the ghost-plane owner tile (incoming chunks)
     int slot;        // slot within the face
     int pop;         // population
-    int u0, v0;      // first cell in face coordinates of the owner tile
-    int nu, nv;      // extent (nu * nv <= 64)
-    int lds0;        // LDS index (floats) of cell (u0, v0)
-    int su, sv;      // LDS strides along u and v
+    int w0;          // first row of the window
+    int rows;        // rows of the window (rows * Bu <= 64, a power of two)
+    int ulo, uhi;    // cells of a row that are moved
+    int vlo, vhi;    // rows that are moved (within the window)
+    int lds0;        // LDS index (floats) of cell (0, w0) under the map below
+    int su, sv;      // LDS index = lds0 + u * su + (v - w0) * sv
 };
 
 template <int BX, int BY, int BZ>
@@ -47,6 +55,38 @@ __host__ __device__ constexpr int face_slot(int face, int pop, int qf) {
 
 __host__ __device__ constexpr int cvec(int i, int a) { return a == 0 ? cxv(i) : (a == 1 ? cyv(i) : czv(i)); }
 
+// face coordinate axes of a face on axis a: u is x unless the face is an
+// x face (then y); v is z unless the face is a z face (then y)
+__host__ __device__ constexpr int face_u_axis(int a) { return a == 0 ? 1 : 0; }
+__host__ __device__ constexpr int face_v_axis(int a) { return a == 2 ? 1 : 2; }
+
+template <int BX, int BY, int BZ>
+__host__ __device__ constexpr int face_bu(int face) {
+    return HaloGeom<BX, BY, BZ>::B[face_u_axis(face / 2)];
+}
+template <int BX, int BY, int BZ>
+__host__ __device__ constexpr int face_bv(int face) {
+    return HaloGeom<BX, BY, BZ>::B[face_v_axis(face / 2)];
+}
+template <int BX, int BY, int BZ>
+__host__ __device__ constexpr int window_rows(int face) {
+    const int bu = face_bu<BX, BY, BZ>(face), bv = face_bv<BX, BY, BZ>(face);
+    const int r = 64 / bu > 0 ? 64 / bu : 1;
+    return r < bv ? r : bv;
+}
+
+// Split rows [vlo, vhi] of a face into windows; calls emit(w0, vlo', vhi').
+template <typename F>
+constexpr void for_windows(int bv, int rows, int vlo, int vhi, F&& emit) {
+    int w = vlo;
+    while (w <= vhi) {
+        const int w0 = w + rows <= bv ? w : bv - rows;
+        const int hi = (w0 + rows - 1) < vhi ? (w0 + rows - 1) : vhi;
+        emit(w0, w, hi);
+        w = hi + 1;
+    }
+}
+
 // Incoming chunks of population i (0 < i < Q). Called twice: with out ==
 // nullptr to count, then to fill.
 template <int Q, int BX, int BY, int BZ>
@@ -62,8 +102,10 @@ constexpr int incoming_chunks_of(int i, Chunk* out) {
         if (!ok) continue;
         const int win = (A & 4) ? 2 : ((A & 2) ? 1 : 0);
         const int face = 2 * win + (c[win] > 0 ? 1 : 0);
-        const int ua = win == 0 ? 1 : 0, va = win == 2 ? 1 : 2;  // face coordinate axes
-        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, e0[3] = {0, 0, 0}, d[3] = {0, 0, 0};
+        const int ua = face_u_axis(win), va = face_v_axis(win);
+        // per axis: source range in the owner tile, LDS cell of its first
+        // source, and whether the LDS cell moves with the source
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, e0[3] = {0, 0, 0}, mv[3] = {0, 0, 0}, d[3] = {0, 0, 0};
         for (int a = 0; a < 3; a++) {
             const int Ba = G::B[a];
             if ((A >> a) & 1) {
@@ -73,16 +115,19 @@ constexpr int incoming_chunks_of(int i, Chunk* out) {
                 e0[a] = c[a] > 0 ? 0 : Ba;
                 d[a] = -c[a];
             } else {
-                // inside: source range whose destination stays in the tile
+                // inside: sources whose destination stays in the tile
                 lo[a] = c[a] < 0 ? 1 : 0;
                 hi[a] = c[a] > 0 ? Ba - 2 : Ba - 1;
-                e0[a] = lo[a] + (c[a] > 0 ? 1 : 0);
+                e0[a] = c[a] > 0 ? 1 : 0;  // LDS cell = source + [c > 0]
+                mv[a] = 1;
             }
         }
-        const int nu = hi[ua] - lo[ua] + 1, nvt = hi[va] - lo[va] + 1;
-        if (nu <= 0 || nvt <= 0) continue;
-        const int vstep = 64 / nu > 0 ? 64 / nu : 1;
-        for (int v = 0; v < nvt; v += vstep) {
+        if (hi[ua] < lo[ua] || hi[va] < lo[va]) continue;
+        const int bv = G::B[va];
+        // a single fixed row is read alone (a taller window would fetch
+        // lines no cell needs)
+        const int rows = mv[va] ? window_rows<BX, BY, BZ>(face) : 1;
+        for_windows(bv, rows, lo[va], hi[va], [&](int w0, int vlo, int vhi) {
             if (out) {
                 Chunk ch{};
                 ch.face = face;
@@ -91,19 +136,22 @@ constexpr int incoming_chunks_of(int i, Chunk* out) {
                 ch.dz = d[2];
                 ch.slot = face_slot(face, i, QF);
                 ch.pop = i;
-                ch.u0 = lo[ua];
-                ch.v0 = lo[va] + v;
-                ch.nu = nu;
-                ch.nv = (nvt - v) < vstep ? (nvt - v) : vstep;
+                ch.w0 = w0;
+                ch.rows = rows;
+                ch.ulo = lo[ua];
+                ch.uhi = hi[ua];
+                ch.vlo = vlo;
+                ch.vhi = vhi;
+                ch.su = mv[ua] ? G::STRIDE[ua] : 0;
+                ch.sv = mv[va] ? G::STRIDE[va] : 0;
+                // LDS cell of face cell (0, w0): moving axes at source 0 / w0
                 int e[3] = {e0[0], e0[1], e0[2]};
-                e[va] += v;
+                if (mv[va]) e[va] += w0;
                 ch.lds0 = (i - 1) * G::E + e[0] + G::DX * (e[1] + G::DY * e[2]);
-                ch.su = G::STRIDE[ua];
-                ch.sv = G::STRIDE[va];
                 out[n] = ch;
             }
             n++;
-        }
+        });
     }
     return n;
 }
@@ -132,25 +180,27 @@ constexpr int outgoing_chunks_impl(Chunk* out) {
     int n = 0;
     for (int face = 0; face < 6; face++) {
         const int a = face / 2;
-        const int ua = a == 0 ? 1 : 0, va = a == 2 ? 1 : 2;
+        const int ua = face_u_axis(a), va = face_v_axis(a);
+        const int bv = G::B[va];
+        const int rows = window_rows<BX, BY, BZ>(face);
         for (int s = 0; s < QF; s++) {
             const int i = face_pop(face, s);
             const int c[3] = {cvec(i, 0), cvec(i, 1), cvec(i, 2)};
-            const int nu = G::B[ua], nvt = G::B[va];
-            const int vstep = 64 / nu > 0 ? 64 / nu : 1;
-            for (int v = 0; v < nvt; v += vstep) {
+            for_windows(bv, rows, 0, bv - 1, [&](int w0, int vlo, int vhi) {
                 if (out) {
                     Chunk ch{};
                     ch.face = face;
                     ch.slot = s;
                     ch.pop = i;
-                    ch.u0 = 0;
-                    ch.v0 = v;
-                    ch.nu = nu;
-                    ch.nv = (nvt - v) < vstep ? (nvt - v) : vstep;
+                    ch.w0 = w0;
+                    ch.rows = rows;
+                    ch.ulo = 0;
+                    ch.uhi = G::B[ua] - 1;
+                    ch.vlo = vlo;
+                    ch.vhi = vhi;
                     int node[3] = {0, 0, 0};
                     node[a] = (face & 1) ? G::B[a] - 1 : 0;
-                    node[va] = v;
+                    node[va] = w0;
                     int e[3] = {0, 0, 0};
                     for (int b = 0; b < 3; b++) e[b] = node[b] + (c[b] > 0 ? 1 : 0);
                     ch.lds0 = (i - 1) * G::E + e[0] + G::DX * (e[1] + G::DY * e[2]);
@@ -159,7 +209,7 @@ constexpr int outgoing_chunks_impl(Chunk* out) {
                     out[n] = ch;
                 }
                 n++;
-            }
+            });
         }
     }
     return n;
@@ -176,5 +226,12 @@ constexpr std::array<Chunk, n_outgoing_chunks<Q, BX, BY, BZ>()> outgoing_chunks(
     outgoing_chunks_impl<Q, BX, BY, BZ>(&arr[0]);
     return arr;
 }
+
+// Both tables of a tile shape, evaluated once per shape.
+template <int Q, int BX, int BY, int BZ>
+struct ChunkTables {
+    static constexpr auto in = incoming_chunks<Q, BX, BY, BZ>();
+    static constexpr auto out = outgoing_chunks<Q, BX, BY, BZ>();
+};
 
 }  // namespace mrlbm

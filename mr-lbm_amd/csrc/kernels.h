@@ -120,8 +120,10 @@ hipError_t launch_step_batch(int q, int bx, int by, int bz, int storage, bool ha
 hipError_t launch_batch_advance(InstArgs* tab, int ninst, int nsteps, hipStream_t s);
 
 // Launchers (step_tile.hip). Return hipError_t of the launch.
+// min(grid, ntiles) workgroups walking tiles blockIdx.x, + gridDim.x, ...;
+// grid <= 0: one tile per workgroup.
 hipError_t launch_step_tile(int q, int bx, int by, int bz, bool has_bc, int storage, const StepArgs& a,
-                            int ntiles, hipStream_t s);
+                            int ntiles, int grid, hipStream_t s);
 bool tile_shape_supported(int q, int bx, int by, int bz, int storage);
 
 // Column kernel (step_column.hip): one (x,y) column per thread over a

@@ -34,41 +34,50 @@ struct Nbr {
 };
 
 // Ghost value of incoming chunk K for this lane. Every lane loads (lanes
-// past the chunk re-read its first cell, a line the chunk reads anyway), so
-// a wave's loads of all its chunks issue back to back with no exec-mask
-// branch between them and one wait covers them all.
+// past the window re-read it), so a wave's loads of all its chunks issue
+// back to back with no exec-mask branch between them and one wait covers
+// them all. The address is the owner's plane base (uniform) + lane + a
+// constant: no per-lane index arithmetic.
 template <int Q, int BX, int BY, int BZ, int K>
 __device__ __forceinline__ float halo_fetch(const float* const* gin, int lane, const Nbr& nb) {
-    constexpr auto CH = incoming_chunks<Q, BX, BY, BZ>();
-    constexpr Chunk c = CH[K];
+    constexpr Chunk c = ChunkTables<Q, BX, BY, BZ>::in[K];
     constexpr int QF = VS<Q>::QF;
-    constexpr int Bu = c.face < 2 ? BY : BX;
-    constexpr int Bv = c.face < 4 ? BZ : BY;
+    constexpr int Bu = face_bu<BX, BY, BZ>(c.face);
+    constexpr int Bv = face_bv<BX, BY, BZ>(c.face);
+    constexpr int NC = Bu * c.rows;
+    static_assert(64 % Bu == 0 && (NC & (NC - 1)) == 0, "face rows must tile a wave");
     const int otile = nb.tile(c.dx, c.dy, c.dz);
-    const auto src = gptr(gin[c.face]) + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)otile);
-    const int lu = lane % c.nu, lv = lane / c.nu;
-    const uint32_t idx = lane < c.nu * c.nv ? (uint32_t)((c.u0 + lu) + Bu * (c.v0 + lv)) : (uint32_t)(c.u0 + Bu * c.v0);
-    return src[idx];
+    const auto src = gptr(gin[c.face]) + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)otile) + Bu * c.w0;
+    return src[NC >= 64 ? (uint32_t)lane : ((uint32_t)lane & (NC - 1))];
+}
+
+// Face cell (u, row r of the window) of this lane in chunk K of the
+// incoming (IN) or outgoing table; false if the lane moves no cell.
+template <int Q, int BX, int BY, int BZ, bool IN, int K>
+__device__ __forceinline__ bool chunk_cell(int lane, int& u, int& r) {
+    using CT = ChunkTables<Q, BX, BY, BZ>;
+    constexpr Chunk c = IN ? CT::in[K] : CT::out[K];
+    constexpr int Bu = face_bu<BX, BY, BZ>(c.face);
+    u = lane % Bu;
+    r = lane / Bu;
+    const int v = c.w0 + r;
+    return (Bu * c.rows >= 64 || lane < Bu * c.rows) && u >= c.ulo && u <= c.uhi && v >= c.vlo && v <= c.vhi;
 }
 
 template <int Q, int BX, int BY, int BZ, int K>
 __device__ __forceinline__ void halo_put(int lane, float v, float* s) {
-    constexpr auto CH = incoming_chunks<Q, BX, BY, BZ>();
-    constexpr Chunk c = CH[K];
-    if (lane < c.nu * c.nv) {
-        const int lu = lane % c.nu, lv = lane / c.nu;
-        s[c.lds0 + lu * c.su + lv * c.sv] = v;
-    }
+    constexpr Chunk c = ChunkTables<Q, BX, BY, BZ>::in[K];
+    int u, r;
+    if (chunk_cell<Q, BX, BY, BZ, true, K>(lane, u, r)) s[c.lds0 + u * c.su + r * c.sv] = v;
 }
 
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int K>
 __device__ __forceinline__ void halo_store(const StepArgs& a, float* const* gout, int lane, int tile, int bx, int by,
                                            int bz, const float* s, const uint8_t* s_cls) {
-    constexpr auto CH = outgoing_chunks<Q, BX, BY, BZ>();
-    constexpr Chunk c = CH[K];
+    constexpr Chunk c = ChunkTables<Q, BX, BY, BZ>::out[K];
     constexpr int QF = VS<Q>::QF;
-    constexpr int Bu = c.face < 2 ? BY : BX;
-    constexpr int Bv = c.face < 4 ? BZ : BY;
+    constexpr int Bu = face_bu<BX, BY, BZ>(c.face);
+    constexpr int Bv = face_bv<BX, BY, BZ>(c.face);
     // wall predicate of src/includeFiles/interface.h:6-47: a tile face on a
     // wall-axis domain boundary is not written
     bool skip = false;
@@ -79,20 +88,20 @@ __device__ __forceinline__ void halo_store(const StepArgs& a, float* const* gout
     if constexpr (c.face == 4) skip = a.wall_z && bz == 0;
     if constexpr (c.face == 5) skip = a.wall_z && bz == a.ntz - 1;
     if (skip) return;
-    const auto dst = gptr(gout[c.face]) + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)tile);
-    if (lane < c.nu * c.nv) {
-        const int lu = lane % c.nu, lv = lane / c.nu;
-        const int u = c.u0 + lu, v = c.v0 + lv;
+    const auto dst = gptr(gout[c.face]) + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)tile) + Bu * c.w0;
+    int u, r;
+    if (chunk_cell<Q, BX, BY, BZ, false, K>(lane, u, r)) {
         if constexpr (HAS_BC) {
             // node of the face cell (solid nodes write no ghost values)
             constexpr int ax = c.face / 2;
             const int fix = (c.face & 1) ? (ax == 0 ? BX : (ax == 1 ? BY : BZ)) - 1 : 0;
+            const int v = c.w0 + r;
             const int nx = ax == 0 ? fix : u;
             const int ny = ax == 1 ? fix : (ax == 0 ? u : v);
             const int nz = ax == 2 ? fix : v;
             if (s_cls[nx + BX * (ny + BY * nz)] == 255u) return;
         }
-        dst[(uint32_t)(u + Bu * v)] = s[c.lds0 + lu * c.su + lv * c.sv];
+        dst[(uint32_t)lane] = s[c.lds0 + u * c.su + r * c.sv];
     }
 }
 
@@ -103,61 +112,117 @@ struct TileLds {
     uint8_t cls[BX * BY * BZ];
 };
 
-// One time step of one tile (the body of k_step_tile and of the batched
-// kernel). The whole workgroup calls it with the same tile.
-template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
-__device__ __forceinline__ void tile_step(const StepArgs& a, const float* const* gin, float* const* gout, uint32_t step,
-                                          int tile, TileLds<Q, BX, BY, BZ>& L) {
-    constexpr int T = BX * BY * BZ;
-    constexpr int NW = T / 64;
-    using G = HaloGeom<BX, BY, BZ>;
-    constexpr int E = G::E;
-    constexpr int NIN = n_incoming_chunks<Q, BX, BY, BZ>();
-    constexpr int NOUT = n_outgoing_chunks<Q, BX, BY, BZ>();
-    float* s = L.s;
-    uint8_t* s_cls = L.cls;
+// HBM reads of one tile update (class bytes, stored moments, this wave's
+// ghost chunks), issued ahead of the tile's computation: a workgroup that
+// walks several tiles loads tile k+1 while it computes tile k.
+template <int Q, int BX, int BY, int BZ, int S>
+struct TileIn {
+    static constexpr int NW = BX * BY * BZ / 64;
+    static constexpr int NIN = (int)ChunkTables<Q, BX, BY, BZ>::in.size();
+    static constexpr int RMAX = (NIN + NW - 1) / NW;
+    typename Codec<S>::T m[10];
+    float h[RMAX];
+    uint32_t cls;
+};
 
-    const int t = threadIdx.x;
-    const int tx = t % BX, ty = (t / BX) % BY, tz = t / (BX * BY);
-    const int lane = t % 64;
-    const int w = __builtin_amdgcn_readfirstlane(t / 64);
-    const int bx = tile % a.ntx, by = (tile / a.ntx) % a.nty, bz = tile / (a.ntx * a.nty);
-    const int base = tx + G::DX * (ty + G::DY * tz);
-    // periodic tile neighbours, as the reference's bxm1/bxp1 (src/mlbm.cu:115-122)
-    Nbr nb;
+// threadIdx.x as a value the compiler cannot see through: per-lane index
+// arithmetic derived from it is recomputed in each tile of a tile loop
+// instead of being hoisted out of the loop and held in registers (which
+// would cost more occupancy than the arithmetic costs issue slots).
+template <int T>
+__device__ __forceinline__ int thread_index() {
+    uint32_t t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return (int)(t & (uint32_t)(T - 1));  // T is a power of two: 0 <= t < T stays known
+}
+
+// Tile coordinates of a tile index and its periodic neighbours, as the
+// reference's bxm1/bxp1 (src/mlbm.cu:115-122).
+__device__ __forceinline__ void tile_coords(const StepArgs& a, int tile, int& bx, int& by, int& bz, Nbr& nb) {
+    bx = tile % a.ntx;
+    by = (tile / a.ntx) % a.nty;
+    bz = tile / (a.ntx * a.nty);
     nb.ntx = a.ntx;
     nb.nty = a.nty;
     nb.x[0] = bx == 0 ? a.ntx - 1 : bx - 1, nb.x[1] = bx, nb.x[2] = bx == a.ntx - 1 ? 0 : bx + 1;
     nb.y[0] = by == 0 ? a.nty - 1 : by - 1, nb.y[1] = by, nb.y[2] = by == a.nty - 1 ? 0 : by + 1;
     nb.z[0] = bz == 0 ? a.ntz - 1 : bz - 1, nb.z[1] = bz, nb.z[2] = bz == a.ntz - 1 ? 0 : bz + 1;
+}
 
-    uint32_t cls = 0;
+template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
+__device__ __forceinline__ void tile_fetch(const StepArgs& a, const float* const* gin, int tile,
+                                           TileIn<Q, BX, BY, BZ, S>& in) {
+    constexpr int T = BX * BY * BZ;
+    constexpr int NW = T / 64;
+    constexpr int NIN = TileIn<Q, BX, BY, BZ, S>::NIN;
+    const int t = thread_index<T>();
+    const int lane = t % 64;
+    const int w = __builtin_amdgcn_readfirstlane(t / 64);
+    int bx, by, bz;
+    Nbr nb;
+    tile_coords(a, tile, bx, by, bz, nb);
+    in.cls = 0;
     if constexpr (HAS_BC) {
-        if (a.cls) cls = gptr(a.cls)[(size_t)tile * T + t];  // uniform: batched lattices may have no class map
-        s_cls[t] = (uint8_t)cls;
+        if (a.cls) in.cls = gptr(a.cls)[(size_t)tile * T + t];  // uniform: batched lattices may have no class map
     }
+    const auto mom = gptr((const typename Codec<S>::T*)a.mom);
+    const size_t mb = (size_t)tile * 10 * T + t;
+#pragma unroll
+    for (int k = 0; k < 10; k++) in.m[k] = mom[mb + (size_t)k * T];
+    // wave W owns incoming chunks W, W + NW, ...; every wave fills all
+    // RMAX slots (zeros past its last chunk) so that no slot index depends
+    // on the wave (a wave-dependent index would move the array to scratch)
+    constexpr int RMAX = TileIn<Q, BX, BY, BZ, S>::RMAX;
+    dispatch<NW>(w, [&](auto WW) {
+        constexpr int W = decltype(WW)::value;
+        static_for<RMAX>([&](auto RR) {
+            constexpr int r = decltype(RR)::value;
+            if constexpr (W + NW * r < NIN)
+                in.h[r] = halo_fetch<Q, BX, BY, BZ, W + NW * r>(gin, lane, nb);
+            else
+                in.h[r] = 0.0f;
+        });
+    });
+}
+
+// One time step of one tile from its fetched inputs (the body of
+// k_step_tile and of the batched kernel). The whole workgroup calls it
+// with the same tile. `ahead` is called once the inputs are consumed
+// (before the first barrier): it issues the next tile's fetch.
+template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S, typename Ahead>
+__device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gout, uint32_t step, int tile,
+                                            TileLds<Q, BX, BY, BZ>& L, const TileIn<Q, BX, BY, BZ, S>& in,
+                                            Ahead&& ahead) {
+    constexpr int T = BX * BY * BZ;
+    constexpr int NW = T / 64;
+    using G = HaloGeom<BX, BY, BZ>;
+    constexpr int E = G::E;
+    constexpr int NIN = TileIn<Q, BX, BY, BZ, S>::NIN;
+    constexpr int NOUT = (int)ChunkTables<Q, BX, BY, BZ>::out.size();
+    float* s = L.s;
+    uint8_t* s_cls = L.cls;
+
+    const int t = thread_index<T>();
+    const int tx = t % BX, ty = (t / BX) % BY, tz = t / (BX * BY);
+    const int lane = t % 64;
+    const int w = __builtin_amdgcn_readfirstlane(t / 64);
+    const int bx = tile % a.ntx, by = (tile / a.ntx) % a.nty, bz = tile / (a.ntx * a.nty);
+    const int base = tx + G::DX * (ty + G::DY * tz);
+
+    const uint32_t cls = in.cls;
+    if constexpr (HAS_BC) s_cls[t] = (uint8_t)cls;
     const bool solid = HAS_BC && (cls == 255u);
 
-    const size_t mb = (size_t)tile * 10 * T + t;
     Node n;
-    load_node<Q, BX, BY, BZ, S>(a, mb, n);
-
-    // populations entering the tile: neighbour ghost planes -> LDS halo;
-    // wave W owns chunks W, W + NW, ...: all loads first, then the writes
-    static_for<NW>([&](auto WW) {
+    decode_node<S>(a, in.m, n);
+    // populations entering the tile: ghost chunks -> LDS halo
+    dispatch<NW>(w, [&](auto WW) {
         constexpr int W = decltype(WW)::value;
         constexpr int R = (NIN - W + NW - 1) / NW;
-        if (w == W) {
-            float hv[R > 0 ? R : 1];
-            static_for<R>([&](auto RR) {
-                constexpr int r = decltype(RR)::value;
-                hv[r] = halo_fetch<Q, BX, BY, BZ, W + NW * r>(gin, lane, nb);
-            });
-            static_for<R>([&](auto RR) {
-                constexpr int r = decltype(RR)::value;
-                halo_put<Q, BX, BY, BZ, W + NW * r>(lane, hv[r], s);
-            });
-        }
+        static_for<R>([&](auto RR) {
+            constexpr int r = decltype(RR)::value;
+            halo_put<Q, BX, BY, BZ, W + NW * r>(lane, in.h[r], s);
+        });
     });
 
     float g[Q];
@@ -167,6 +232,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const float* const*
         constexpr int wo = (cxv(i) > 0) + G::DX * ((cyv(i) > 0) + G::DY * (czv(i) > 0));
         s[(i - 1) * E + base + wo] = g[i];
     });
+    ahead();
     __syncthreads();
     static_for<Q - 1>([&](auto I) {
         constexpr int i = decltype(I)::value + 1;
@@ -186,6 +252,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const float* const*
     const uint32_t node_id = (uint32_t)(bx * BX + tx) + (uint32_t)a.nx * ((uint32_t)(by * BY + ty) + (uint32_t)a.ny * (uint32_t)(bz * BZ + tz));
     quantize<S>(m, a, step, node_id, q);
     reconstruct<Q>(m, g);
+    const size_t mb = (size_t)tile * 10 * T + t;
     if (!solid) store_node<Q, BX, BY, BZ, S>(a, mb, q);
 
     // populations leaving the tile: stage in LDS, then whole waves write
@@ -197,21 +264,37 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const float* const*
         s[(i - 1) * E + base + wo] = g[i];
     });
     __syncthreads();
-    static_for<NW>([&](auto WW) {
+    dispatch<NW>(w, [&](auto WW) {
         constexpr int W = decltype(WW)::value;
-        if (w == W) {
-            static_for<(NOUT + NW - 1) / NW>([&](auto RR) {
-                constexpr int K = W + NW * decltype(RR)::value;
-                if constexpr (K < NOUT) halo_store<Q, BX, BY, BZ, HAS_BC, K>(a, gout, lane, tile, bx, by, bz, s, s_cls);
-            });
-        }
+        static_for<(NOUT + NW - 1) / NW>([&](auto RR) {
+            constexpr int K = W + NW * decltype(RR)::value;
+            if constexpr (K < NOUT) halo_store<Q, BX, BY, BZ, HAS_BC, K>(a, gout, lane, tile, bx, by, bz, s, s_cls);
+        });
     });
 }
 
+// Tile step over tiles blockIdx.x, + gridDim.x, ...; each tile's reads are
+// issued while the previous tile computes. The library launches one tile
+// per block (measured faster on 256^3 than a resident grid walking tiles
+// with this lookahead: workgroup turnover hides the load latency better).
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
-__global__ __launch_bounds__(BX* BY* BZ) void k_step_tile(StepArgs a) {
+__global__ __launch_bounds__(BX* BY* BZ) void k_step_tile(StepArgs a, int ntiles) {
     __shared__ TileLds<Q, BX, BY, BZ> L;
-    tile_step<Q, BX, BY, BZ, HAS_BC, S>(a, a.gin, a.gout, a.step, blockIdx.x, L);
+    using In = TileIn<Q, BX, BY, BZ, S>;
+    int tile = blockIdx.x;
+    In in;
+    tile_fetch<Q, BX, BY, BZ, HAS_BC, S>(a, a.gin, tile, in);
+    while (true) {
+        const int next = tile + (int)gridDim.x;
+        In nx;
+        tile_update<Q, BX, BY, BZ, HAS_BC, S>(a, a.gout, a.step, tile, L, in, [&]() {
+            if (next < ntiles) tile_fetch<Q, BX, BY, BZ, HAS_BC, S>(a, a.gin, next, nx);
+        });
+        if (next >= ntiles) break;
+        __syncthreads();  // LDS is reused by the next tile
+        in = nx;
+        tile = next;
+    }
 }
 
 // Batched step: each block takes a contiguous range of work items (the
@@ -223,16 +306,39 @@ template <int Q, int BX, int BY, int BZ, int S, bool HAS_BC>
 __global__ __launch_bounds__(BX* BY* BZ) void k_step_batch(const InstArgs* __restrict__ tab,
                                                            const uint32_t* __restrict__ work, int nwork, int k) {
     __shared__ TileLds<Q, BX, BY, BZ> L;
+    using In = TileIn<Q, BX, BY, BZ, S>;
     const int per = (nwork + (int)gridDim.x - 1) / (int)gridDim.x;
     const int begin = (int)blockIdx.x * per;
     const int end = begin + per < nwork ? begin + per : nwork;
-    for (int item = begin; item < end; item++) {
+    if (begin >= end) return;
+    auto item_of = [&](int item, int& inst, int& tile) {
         const uint32_t wv = __builtin_amdgcn_readfirstlane(work[item]);
-        const int inst = (int)(wv >> 20), tile = (int)(wv & 0xFFFFFu);
+        inst = (int)(wv >> 20);
+        tile = (int)(wv & 0xFFFFFu);
+    };
+    int inst, tile;
+    item_of(begin, inst, tile);
+    In in;
+    {
         const InstArgs& ia = tab[inst];  // read in place (scalar loads)
+        tile_fetch<Q, BX, BY, BZ, HAS_BC, S>(ia.a, ia.g[(ia.cur0 + k) & 1], tile, in);
+    }
+    for (int item = begin; item < end; item++) {
+        const InstArgs& ia = tab[inst];
         const int p = (ia.cur0 + k) & 1;
-        tile_step<Q, BX, BY, BZ, HAS_BC, S>(ia.a, ia.g[p], ia.g[1 - p], ia.step0 + (uint32_t)k, tile, L);
+        int ninst = inst, ntile = tile;
+        if (item + 1 < end) item_of(item + 1, ninst, ntile);
+        In nx;
+        tile_update<Q, BX, BY, BZ, HAS_BC, S>(ia.a, ia.g[1 - p], ia.step0 + (uint32_t)k, tile, L, in, [&]() {
+            if (item + 1 < end) {
+                const InstArgs& nia = tab[ninst];
+                tile_fetch<Q, BX, BY, BZ, HAS_BC, S>(nia.a, nia.g[(nia.cur0 + k) & 1], ntile, nx);
+            }
+        });
         __syncthreads();  // LDS is reused by the next item
+        in = nx;
+        inst = ninst;
+        tile = ntile;
     }
 }
 
@@ -289,30 +395,34 @@ bool tile_shape_supported(int q, int bx, int by, int bz, int storage) {
     return false;
 }
 
-template <int Q, int BX, int BY, int BZ, int S>
-static hipError_t launch_qs(bool has_bc, const StepArgs& a, int ntiles, hipStream_t s) {
-    dim3 grid(ntiles), block(BX * BY * BZ);
-    if (has_bc)
-        hipLaunchKernelGGL((k_step_tile<Q, BX, BY, BZ, true, S>), grid, block, 0, s, a);
-    else
-        hipLaunchKernelGGL((k_step_tile<Q, BX, BY, BZ, false, S>), grid, block, 0, s, a);
+template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
+static hipError_t launch_qsb(const StepArgs& a, int ntiles, int grid, hipStream_t s) {
+    constexpr int T = BX * BY * BZ;
+    if (grid <= 0 || grid > ntiles) grid = ntiles;
+    hipLaunchKernelGGL((k_step_tile<Q, BX, BY, BZ, HAS_BC, S>), dim3(grid), dim3(T), 0, s, a, ntiles);
     return hipGetLastError();
 }
 
+template <int Q, int BX, int BY, int BZ, int S>
+static hipError_t launch_qs(bool has_bc, const StepArgs& a, int ntiles, int grid, hipStream_t s) {
+    return has_bc ? launch_qsb<Q, BX, BY, BZ, true, S>(a, ntiles, grid, s)
+                  : launch_qsb<Q, BX, BY, BZ, false, S>(a, ntiles, grid, s);
+}
+
 template <int Q, int BX, int BY, int BZ>
-static hipError_t launch_q_shape(bool has_bc, int storage, const StepArgs& a, int ntiles, hipStream_t s) {
-    if (storage == 0) return launch_qs<Q, BX, BY, BZ, 0>(has_bc, a, ntiles, s);
-    if (storage == 1) return launch_qs<Q, BX, BY, BZ, 1>(has_bc, a, ntiles, s);
-    if (storage == 2) return launch_qs<Q, BX, BY, BZ, 2>(has_bc, a, ntiles, s);
+static hipError_t launch_q_shape(bool has_bc, int storage, const StepArgs& a, int ntiles, int grid, hipStream_t s) {
+    if (storage == 0) return launch_qs<Q, BX, BY, BZ, 0>(has_bc, a, ntiles, grid, s);
+    if (storage == 1) return launch_qs<Q, BX, BY, BZ, 1>(has_bc, a, ntiles, grid, s);
+    if (storage == 2) return launch_qs<Q, BX, BY, BZ, 2>(has_bc, a, ntiles, grid, s);
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_step_tile(int q, int bx, int by, int bz, bool has_bc, int storage, const StepArgs& a, int ntiles,
-                            hipStream_t s) {
-#define X(A, B, C)                                                                    \
-    if (bx == A && by == B && bz == C) {                                              \
-        if (q == 19) return launch_q_shape<19, A, B, C>(has_bc, storage, a, ntiles, s); \
-        if (q == 27) return launch_q_shape<27, A, B, C>(has_bc, storage, a, ntiles, s); \
+                            int grid, hipStream_t s) {
+#define X(A, B, C)                                                                          \
+    if (bx == A && by == B && bz == C) {                                                    \
+        if (q == 19) return launch_q_shape<19, A, B, C>(has_bc, storage, a, ntiles, grid, s); \
+        if (q == 27) return launch_q_shape<27, A, B, C>(has_bc, storage, a, ntiles, grid, s); \
     }
     MRLBM_TILE_SHAPES(X)
 #undef X

@@ -42,6 +42,23 @@ __device__ __forceinline__ void load_node(const StepArgs& a, size_t mb, Node& n)
     n.mzz = C::ld(mom, mb + 9 * T, a.mrscale[9]);
 }
 
+// Node moments from the raw stored values (fetched earlier).
+template <int S>
+__device__ __forceinline__ void decode_node(const StepArgs& a, const typename Codec<S>::T* m, Node& n) {
+    using C = Codec<S>;
+    n.drho = C::dec(m[0], a.mrscale[0]);
+    n.rho = a.rho0 + n.drho;
+    n.ux = C::dec(m[1], a.mrscale[1]);
+    n.uy = C::dec(m[2], a.mrscale[2]);
+    n.uz = C::dec(m[3], a.mrscale[3]);
+    n.mxx = C::dec(m[4], a.mrscale[4]);
+    n.mxy = C::dec(m[5], a.mrscale[5]);
+    n.mxz = C::dec(m[6], a.mrscale[6]);
+    n.myy = C::dec(m[7], a.mrscale[7]);
+    n.myz = C::dec(m[8], a.mrscale[8]);
+    n.mzz = C::dec(m[9], a.mrscale[9]);
+}
+
 // Round the post-collision moments to the storage format and replace them
 // by the values the storage will hold, so the populations sent through the
 // ghost planes are the ones the neighbour would reconstruct from the stored

@@ -156,3 +156,20 @@ def test_spheres_voxelisation():
     s = geometry.spheres((16, 16, 16), [(8, 8, 8)], [3.0])
     x, y, z = np.meshgrid(*[np.arange(16)] * 3, indexing="ij")
     np.testing.assert_array_equal(s, (x - 8) ** 2 + (y - 8) ** 2 + (z - 8) ** 2 <= 9)
+
+
+def test_halo_chunk_tables(tmp_path):
+    """The tile kernel's ghost-chunk tables move every halo cell once from
+    the right ghost-plane cell and read only inside the owner's plane
+    (host simulation of every lane, tests/halo_check.cpp)."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    exe = tmp_path / "halo_check"
+    src = os.path.join(os.path.dirname(__file__), "halo_check.cpp")
+    subprocess.run([hipcc, "-std=c++17", "-O1", "--offload-host-only", "-x", "hip", src, "-o", str(exe)],
+                   check=True, capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout
