@@ -64,6 +64,7 @@ int mrlbm_destroy(mrlbm_ctx* c) {
     hipFree(c->d_mom);
     hipFree(c->d_cls);
     hipFree(c->d_bc);
+    hipFree(c->d_tiles);
     for (int s = 0; s < 2; s++)
         for (int f = 0; f < 6; f++) hipFree(c->d_ghost[s][f]);
     hipFree(c->d_partial);
@@ -209,7 +210,7 @@ int mrlbm_set_node_types(mrlbm_ctx* c, const uint32_t* types) {
     c->n_classes = (int)tab.keys.size() - 1;
     c->version++;
     c->cls_version++;
-    if (!c->has_bc) return MRLBM_OK;
+    if (!c->has_bc) return build_tile_list(c);
     std::vector<uint8_t> blk(c->N);
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
     for (int z = 0; z < nz; z++)
@@ -219,7 +220,7 @@ int mrlbm_set_node_types(mrlbm_ctx* c, const uint32_t* types) {
     HIPCHK(hipMalloc((void**)&c->d_bc, tab.table.size() * sizeof(float)));
     HIPCHK(hipMemcpy(c->d_cls, blk.data(), c->N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_bc, tab.table.data(), tab.table.size() * sizeof(float), hipMemcpyHostToDevice));
-    return MRLBM_OK;
+    return build_tile_list(c);
 }
 
 }  // extern "C"
@@ -414,12 +415,24 @@ int mrlbm_init_taylor_green(mrlbm_ctx* c, double u0, int mode) {
 }
 
 static int do_steps(mrlbm_ctx* c, int nsteps) {
+    const bool split = c->p.kernel == MRLBM_KERNEL_TILE && c->has_bc && c->d_tiles;
     for (int s = 0; s < nsteps; s++) {
         StepArgs a = c->args();
-        hipError_t e =
-            c->p.kernel == MRLBM_KERNEL_COLUMN
-                ? launch_step_column(c->p.q, c->bx, c->by, c->bz, c->has_bc, c->p.storage, a, c->ntiles, c->stream)
-                : launch_step_tile(c->p.q, c->bx, c->by, c->bz, c->has_bc, c->p.storage, a, c->ntiles, 0, c->stream);
+        hipError_t e;
+        if (c->p.kernel == MRLBM_KERNEL_COLUMN) {
+            e = launch_step_column(c->p.q, c->bx, c->by, c->bz, c->has_bc, c->p.storage, a, c->ntiles, c->stream);
+        } else if (split) {
+            // all-bulk tiles with the light body, the rest with boundaries;
+            // both read the same ghost set and write disjoint data
+            e = launch_step_tile(c->p.q, c->bx, c->by, c->bz, false, c->p.storage, a, c->d_tiles, c->n_bulk_tiles, 0,
+                                 c->stream);
+            if (e == hipSuccess)
+                e = launch_step_tile(c->p.q, c->bx, c->by, c->bz, true, c->p.storage, a, c->d_tiles + c->n_bulk_tiles,
+                                     c->ntiles - c->n_bulk_tiles, 0, c->stream);
+        } else {
+            e = launch_step_tile(c->p.q, c->bx, c->by, c->bz, c->has_bc, c->p.storage, a, nullptr, c->ntiles, 0,
+                                 c->stream);
+        }
         if (e != hipSuccess) return fail(MRLBM_EHIP, "step launch failed: %s", hipGetErrorString(e));
         c->cur = 1 - c->cur;
         c->step++;

@@ -52,6 +52,8 @@ struct mrlbm_ctx {
     uint64_t cls_version = 0;        // bumped when the class map is replaced
     std::vector<uint8_t> tile_flags;  // per tile: 1 = holds a boundary or solid node
     uint64_t tile_flags_version = ~0ull;
+    uint32_t* d_tiles = nullptr;      // all-bulk tiles, then the others (tile kernel with boundaries)
+    int n_bulk_tiles = 0;
 
     size_t mom_index(int x, int y, int z, int m) const {
         const int tile = (x / bx) + ntx * ((y / by) + nty * (z / bz));
@@ -119,6 +121,10 @@ int fill_uniform_moments(mrlbm_ctx* c, const uint32_t enc[10]);
 // Per-tile boundary flags of the current class map (computed on the device
 // once per class map).
 int tile_flags(mrlbm_ctx* c, const std::vector<uint8_t>** out);
+
+// Device list of the lattice's tiles, all-bulk ones first (n_bulk_tiles),
+// for the split bulk/boundary launches of a lattice with boundary nodes.
+int build_tile_list(mrlbm_ctx* c);
 
 // The stream a batch launches on (members' work is ordered on it).
 hipStream_t batch_stream(const mrlbm_batch* b);

@@ -205,6 +205,25 @@ int tile_flags(mrlbm_ctx* c, const std::vector<uint8_t>** out) {
     return MRLBM_OK;
 }
 
+int build_tile_list(mrlbm_ctx* c) {
+    c->n_bulk_tiles = c->ntiles;
+    if (!c->has_bc || !c->d_cls) return MRLBM_OK;
+    const std::vector<uint8_t>* flags = nullptr;
+    int rc = tile_flags(c, &flags);
+    if (rc) return rc;
+    std::vector<uint32_t> list;
+    list.reserve(c->ntiles);
+    for (int t = 0; t < c->ntiles; t++)
+        if (!(*flags)[t]) list.push_back((uint32_t)t);
+    c->n_bulk_tiles = (int)list.size();
+    for (int t = 0; t < c->ntiles; t++)
+        if ((*flags)[t]) list.push_back((uint32_t)t);
+    if (!c->d_tiles) HIPCHK(hipMalloc((void**)&c->d_tiles, sizeof(uint32_t) * c->ntiles));
+    HIPCHK(hipMemcpyAsync(c->d_tiles, list.data(), sizeof(uint32_t) * c->ntiles, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MRLBM_OK;
+}
+
 }  // namespace mrlbm
 
 extern "C" {
@@ -283,7 +302,7 @@ int mrlbm_set_geometry(mrlbm_ctx* c, const uint8_t* solid, int nspheres, const d
     c->n_classes = n_classes;
     c->version++;
     c->cls_version++;
-    return MRLBM_OK;
+    return build_tile_list(c);
 }
 
 }  // extern "C"
