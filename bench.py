@@ -47,8 +47,8 @@ def cpu_baseline(target_s):
     threads = _oracle.lib().orc_get_threads()
     o = _oracle.Oracle(n, n, n, q=19, tau=0.53)
     o.init_taylor_green(0.05)
-    t1 = o.step_timed(2)
-    steps = max(2, min(2000, int(target_s / max(t1 / 2, 1e-6))))
+    t1 = o.step_timed(20)  # calibration (also absorbs the thread pool's start-up)
+    steps = max(2, min(5000, int(target_s / max(t1 / 20, 1e-6))))
     t = o.step_timed(steps)
     return {
         "value": n ** 3 * steps / t / 1e6,
@@ -178,14 +178,25 @@ def serving_ducts(n_req=100, rate=20.0, seed=5):
     }
 
 
-def reduced_precision_variants(n, tau, u0, steps, warmup):
-    """Same workload with FP16 and BF16 moment storage (FP32 arithmetic)."""
+def reduced_precision_variants(n, tau, u0, steps, warmup, fp32_mlups):
+    """Same workload with FP16 and BF16 moment storage (FP32 arithmetic);
+    each run's fields are compared with an FP32-storage run of the same
+    step count (per-field relative L2 of rho - 1 and u)."""
+    import numpy as np
     import mrlbm
 
-    res = {}
-    for name, st in (("fp16", mrlbm.STORE_FP16), ("bf16", mrlbm.STORE_BF16)):
+    def fields(st):
         lat = mrlbm.Lattice((n, n, n), q=19, tau=tau, storage=st)
         lat.init_taylor_green(u0)
+        return lat
+
+    ref = fields(mrlbm.STORE_FP32)
+    ref.step(warmup + steps)
+    rho_r, ux_r, uy_r, uz_r = ref.macroscopic()
+    ref.close()
+    res = {}
+    for name, st in (("fp16", mrlbm.STORE_FP16), ("bf16", mrlbm.STORE_BF16)):
+        lat = fields(st)
         info = lat.info()
         lat.step(warmup)
         lat.synchronize()
@@ -193,16 +204,27 @@ def reduced_precision_variants(n, tau, u0, steps, warmup):
         ev_ms = lat.time_steps(steps)
         lat.synchronize()
         t1 = time.perf_counter()
+        rho, ux, uy, uz = lat.macroscopic()
         lat.close()
         avg = ev_ms * 1e-3 / steps
+        mlups = n ** 3 * steps / (t1 - t0) / 1e6
+        achieved = info["bytes_per_lu"] * n ** 3 / avg / 1e9
+
+        def rl2(a, b):
+            return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
         res[name] = {
-            "value": round(n ** 3 * steps / (t1 - t0) / 1e6, 1),
+            "value": round(mlups, 1),
             "unit": "MLUPS",
             "steps": steps,
+            "speedup_vs_fp32": round(mlups / fp32_mlups, 3),
             "tile": [info["bx"], info["by"], info["bz"]],
             "bytes_per_lu": info["bytes_per_lu"],
             "mean_launch_us": round(avg * 1e6, 2),
-            "achieved_gbs": round(info["bytes_per_lu"] * n ** 3 / avg / 1e9, 1),
+            "achieved_gbs": round(achieved, 1),
+            "roofline_frac": round(achieved / PEAK_HBM_GBS, 4),
+            "rel_l2_vs_fp32_storage": {"rho": rl2(rho, rho_r), "rho-1": rl2(rho - 1.0, rho_r - 1.0),
+                                       "u": rl2(np.stack([ux, uy, uz]), np.stack([ux_r, uy_r, uz_r]))},
         }
     return res
 
@@ -218,6 +240,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip the FP16/BF16 storage lines")
+    ap.add_argument("--storage", default="fp32", choices=["fp32", "fp16", "bf16"],
+                    help="moment storage of the headline lattice (arithmetic is FP32 in all)")
+    ap.add_argument("--batch-steps", type=int, default=2000, help="steps of each lattice in the batching comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-cavity batching comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served duct stream")
     args = ap.parse_args()
@@ -235,12 +260,14 @@ def main():
     nu, u0 = 0.01, 0.05
     tau = 0.5 + 3 * nu
     N = n ** 3
-    print(f"bench: D3Q19 TGV {n}^3 kernel {args.kernel} tile {tile} steps {args.steps} warmup {args.warmup}",
+    print(f"bench: D3Q19 TGV {n}^3 {args.storage} kernel {args.kernel} tile {tile} steps {args.steps} "
+          f"warmup {args.warmup}",
           file=sys.stderr, flush=True)
 
     copy_gbs, _ = mrlbm.copy_bandwidth(1 << 30, 20)
 
-    lat = mrlbm.Lattice((n, n, n), q=19, tau=tau, tile=tile, kernel=kernel)
+    storage = {"fp32": mrlbm.STORE_FP32, "fp16": mrlbm.STORE_FP16, "bf16": mrlbm.STORE_BF16}[args.storage]
+    lat = mrlbm.Lattice((n, n, n), q=19, tau=tau, tile=tile, kernel=kernel, storage=storage)
     lat.init_taylor_green(u0)
     info = lat.info()
     tile = (info["bx"], info["by"], info["bz"])
@@ -261,7 +288,7 @@ def main():
     avg_launch_s = ev_ms * 1e-3 / args.steps
     b_lu = info["bytes_per_lu"]
     achieved = b_lu * N / avg_launch_s / 1e9
-    key = f"tgv{n}_d3q19_fp32_{kname}{tile[0]}x{tile[1]}x{tile[2]}"
+    key = f"tgv{n}_d3q19_{args.storage}_{kname}{tile[0]}x{tile[1]}x{tile[2]}"
     traffic = traffic_from_profiles(key)
 
     out = {
@@ -278,9 +305,10 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": f"D3Q19 Taylor-Green vortex {n}^3 triply periodic, FP32 moment storage, "
+            "workload": f"D3Q19 Taylor-Green vortex {n}^3 triply periodic, {args.storage.upper()} moment storage, "
                         f"u0={u0}, nu={nu} (tau={tau}), {kname} kernel {tile[0]}x{tile[1]}x{tile[2]}",
             "lattice": [n, n, n],
+            "moment_storage": args.storage,
             "tile": list(tile),
             "kernel": kname,
             "bytes_per_lu": b_lu,
@@ -304,13 +332,13 @@ def main():
     phase(f"headline {mlups:.1f} MLUPS")
     if not args.no_batch:
         phase("64-cavity batch")
-        out["batching"] = batching_cavities()
+        out["batching"] = batching_cavities(steps=args.batch_steps)
     if not args.no_serve:
         phase("served duct stream")
         out["serving"] = serving_ducts()
     if not args.no_variants:
         phase("FP16/BF16 storage")
-        out["variants"] = reduced_precision_variants(n, tau, u0, args.steps // 2, args.warmup)
+        out["variants"] = reduced_precision_variants(n, tau, u0, args.steps // 2, args.warmup, mlups)
     if not args.no_cpu_baseline:
         phase("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
