@@ -60,11 +60,20 @@ __device__ __forceinline__ void decode_node(const StepArgs& a, const typename Co
 }
 
 // Round the post-collision moments to the storage format and replace them
-// by the values the storage will hold, so the populations sent through the
-// ghost planes are the ones the neighbour would reconstruct from the stored
-// moments: results do not depend on the tile shape. FP32 storage is exact.
-// Stochastic rounding draws 8 random bits per value from three words:
-// one counter-based hash (sr_bits) and two xorshift successors.
+// by the values the storage holds, so the populations a node streams are
+// the ones its neighbours would reconstruct from the stored moments: results
+// do not depend on the tile shape. FP32 storage is exact.
+//
+// Stochastic rounding: v * scale (a power of two, exact) gets uniform random
+// bits added below the kept mantissa bits, then those bits are cleared,
+// giving t (FP16: 10 kept mantissa bits; BF16: 7). BF16 has FP32's exponent
+// range, so t is the stored value exactly: the stored bits are t's upper 16
+// bits and the moment is t / scale. FP16 packs pairs of t by a round-toward-
+// zero conversion (exact for t in the FP16 normal range) and the moments are
+// decoded back from the packed halves, so values below the normal range,
+// which the conversion truncates further, are also used as stored. 8 random
+// bits per value come from three words: one counter-based hash (sr_bits)
+// and two xorshift successors.
 __device__ __forceinline__ uint32_t xs32(uint32_t x) {
     x ^= x << 13;
     x ^= x >> 17;
@@ -75,7 +84,6 @@ __device__ __forceinline__ uint32_t xs32(uint32_t x) {
 template <int S>
 __device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t step, uint32_t node_id,
                                          typename Codec<S>::T* q) {
-    using C = Codec<S>;
     if constexpr (S == 0) {
         q[0] = n.drho;
         q[1] = n.ux;
@@ -90,23 +98,34 @@ __device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t st
     } else {
         const uint32_t w0 = sr_bits(node_id, step, a.seed);
         const uint32_t w1 = xs32(w0), w2 = xs32(w1);
-        constexpr uint32_t SH = S == 1 ? 5u : 8u;            // random byte -> dropped-bit position
-        constexpr uint32_t HALF = S == 1 ? 16u : 128u;       // centre of the byte's interval
-        auto rb = [&](uint32_t w, int k) { return (((w >> (8 * k)) & 0xFFu) << SH) | HALF; };
-#define MRLBM_Q(K, FIELD, W, B)                          \
-    q[K] = C::enc(n.FIELD, a.mscale[K], rb(W, B));       \
-    n.FIELD = C::dec(q[K], a.mrscale[K]);
-        MRLBM_Q(0, drho, w0, 0)
-        MRLBM_Q(1, ux, w0, 1)
-        MRLBM_Q(2, uy, w0, 2)
-        MRLBM_Q(3, uz, w0, 3)
-        MRLBM_Q(4, mxx, w1, 0)
-        MRLBM_Q(5, mxy, w1, 1)
-        MRLBM_Q(6, mxz, w1, 2)
-        MRLBM_Q(7, myy, w1, 3)
-        MRLBM_Q(8, myz, w2, 0)
-        MRLBM_Q(9, mzz, w2, 1)
-#undef MRLBM_Q
+        constexpr uint32_t SH = S == 1 ? 5u : 8u;         // random byte -> dropped-bit position
+        constexpr uint32_t HALF = S == 1 ? 16u : 128u;    // centre of the byte's interval
+        constexpr uint32_t KEEP = S == 1 ? ~0x1FFFu : ~0xFFFFu;
+        float* v[10] = {&n.drho, &n.ux, &n.uy, &n.uz, &n.mxx, &n.mxy, &n.mxz, &n.myy, &n.myz, &n.mzz};
+        const uint32_t w[3] = {w0, w1, w2};
+        float t[10];
+#pragma unroll
+        for (int k = 0; k < 10; k++) {
+            const uint32_t r = (((w[k / 4] >> (8 * (k % 4))) & 0xFFu) << SH) | HALF;
+            t[k] = __uint_as_float((__float_as_uint(*v[k] * a.mscale[k]) + r) & KEEP);
+        }
+        if constexpr (S == 1) {
+#pragma unroll
+            for (int k = 0; k < 10; k += 2) {
+                const auto p = __builtin_amdgcn_cvt_pkrtz(t[k], t[k + 1]);
+                const uint32_t b = __builtin_bit_cast(uint32_t, p);
+                q[k] = (uint16_t)(b & 0xFFFFu);
+                q[k + 1] = (uint16_t)(b >> 16);
+                *v[k] = (float)p[0] * a.mrscale[k];
+                *v[k + 1] = (float)p[1] * a.mrscale[k + 1];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 10; k++) {
+                q[k] = (uint16_t)(__float_as_uint(t[k]) >> 16);
+                *v[k] = t[k] * a.mrscale[k];
+            }
+        }
         n.rho = a.rho0 + n.drho;
     }
 }
