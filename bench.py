@@ -229,6 +229,44 @@ def reduced_precision_variants(n, tau, u0, steps, warmup, fp32_mlups):
     return res
 
 
+def channel_d3q27(steps, shape=(512, 256, 256), g=1e-6, nu=0.05):
+    """D3Q27 force-driven duct (x periodic, walls on y and z) from rest at each
+    storage precision: throughput and the x-averaged streamed velocity against
+    the transient rectangular-duct series at t = steps (and against the
+    FP32-storage run)."""
+    import numpy as np
+    import mrlbm
+    from mrlbm import analytic, geometry
+
+    types = geometry.classify(shape, (0, 1, 1))
+    ser = analytic.duct_start_up(shape[1], shape[2], g, nu, t=steps)
+    out, ref = {}, None
+    for name, st in (("fp32", mrlbm.STORE_FP32), ("fp16", mrlbm.STORE_FP16), ("bf16", mrlbm.STORE_BF16)):
+        with mrlbm.Lattice(shape, q=27, tau=0.5 + 3 * nu, walls=(0, 1, 1), force=(g, 0, 0), storage=st) as lat:
+            lat.set_node_types(types)
+            lat.init_uniform()
+            lat.synchronize()
+            t0 = time.perf_counter()
+            ms = lat.time_steps(steps)
+            lat.synchronize()
+            wall = time.perf_counter() - t0
+            info = lat.info()
+            ux = lat.moments()[1].reshape(shape, order="F").mean(axis=0) / 3.0 - g  # stored u carries +F/rho
+        if ref is None:
+            ref = ux
+        n = float(np.prod(shape))
+        out[name] = {
+            "value": round(n * steps / wall / 1e6, 1),
+            "unit": "MLUPS",
+            "mean_launch_us": round(ms * 1e3 / steps, 2),
+            "bytes_per_lu": info["bytes_per_lu"],
+            "rel_l2_vs_duct_series": float(np.linalg.norm(ux - ser) / np.linalg.norm(ser)),
+            "rel_l2_vs_fp32_storage": float(np.linalg.norm(ux - ref) / np.linalg.norm(ref)),
+        }
+    return {"workload": f"D3Q27 duct {shape[0]}x{shape[1]}x{shape[2]}, x periodic, y/z walls, gx={g}, "
+                        f"nu={nu}, from rest, {steps} steps", **out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,6 +281,8 @@ def main():
     ap.add_argument("--storage", default="fp32", choices=["fp32", "fp16", "bf16"],
                     help="moment storage of the headline lattice (arithmetic is FP32 in all)")
     ap.add_argument("--batch-steps", type=int, default=2000, help="steps of each lattice in the batching comparison")
+    ap.add_argument("--channel-steps", type=int, default=20000, help="steps of the D3Q27 duct at each precision")
+    ap.add_argument("--no-channel", action="store_true", help="skip the D3Q27 duct lines")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-cavity batching comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served duct stream")
     args = ap.parse_args()
@@ -339,6 +379,9 @@ def main():
     if not args.no_variants:
         phase("FP16/BF16 storage")
         out["variants"] = reduced_precision_variants(n, tau, u0, args.steps // 2, args.warmup, mlups)
+    if not args.no_channel:
+        phase("D3Q27 duct at FP32/FP16/BF16")
+        out["channel_d3q27"] = channel_d3q27(args.channel_steps)
     if not args.no_cpu_baseline:
         phase("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

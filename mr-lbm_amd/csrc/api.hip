@@ -537,7 +537,8 @@ int mrlbm_bc_class_info(int q, uint32_t node_type, double omega, const double u[
     if (!u || !known || !ainv49) return fail(MRLBM_EINVAL, "null argument");
     if (q != 19 && q != 27) return fail(MRLBM_EINVAL, "q must be 19 or 27");
     *known = known_mask(q, node_type & 0xFFu);
-    if (!build_bc_inverse(q, *known, omega, u, ainv49)) return fail(MRLBM_ENUMERIC, "singular boundary system");
+    if (!build_bc_inverse(q, *known, omega, u, ainv49, nullptr, closure_lattice(q, node_type & 0xFFu)))
+        return fail(MRLBM_ENUMERIC, "singular boundary system");
     return MRLBM_OK;
 }
 

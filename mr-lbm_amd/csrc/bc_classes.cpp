@@ -50,17 +50,17 @@ uint32_t known_mask(int q, uint32_t octants) {
     return mask;
 }
 
-bool build_bc_inverse(int q, uint32_t known, double omega, const double u[3], double ainv[49], double* cond) {
+bool build_bc_inverse(int q, uint32_t known, double omega, const double u[3], double ainv[49], double* cond,
+                      int closure_q) {
     double A[7][7] = {};
     const double uu[6] = {u[0] * u[0], u[0] * u[1], u[0] * u[2], u[1] * u[1], u[1] * u[2], u[2] * u[2]};
-    // The closure is the D3Q19 system for either lattice (the reference's
-    // D3Q27 wall formulas, cases/parallelPlates_D3Q27/bc_definition.inc:18-72,
-    // carry the D3Q19 coefficients); only the incoming moments summed in the
-    // kernel run over all known D3Q27 populations.
+    // The closure runs over the known populations of lattice closure_q; the
+    // incoming moments summed in the kernel run over all known populations
+    // of the lattice (see closure_lattice).
     (void)q;
-    for (int i = 0; i < 19; i++) {
+    for (int i = 0; i < closure_q; i++) {
         if (!((known >> i) & 1u)) continue;
-        const double w = weight(19, i);
+        const double w = weight(closure_q, i);
         double h[6];
         hermite(i, h);
         const double cu = CX[i] * u[0] + CY[i] * u[1] + CZ[i] * u[2];
@@ -114,10 +114,16 @@ bool build_bc_inverse(int q, uint32_t known, double omega, const double u[3], do
     return true;
 }
 
+int closure_lattice(int q, uint32_t octants) {
+    const bool face = octants == NORTH || octants == SOUTH || octants == WEST || octants == EAST ||
+                      octants == FRONT || octants == BACK;
+    return (q == 27 && face) ? 19 : q;
+}
+
 bool class_record(int q, uint32_t octants, double omega, const double u[3], float* rec) {
     const uint32_t known = known_mask(q, octants);
     double ainv[49];
-    if (!build_bc_inverse(q, known, omega, u, ainv)) return false;
+    if (!build_bc_inverse(q, known, omega, u, ainv, nullptr, closure_lattice(q, octants))) return false;
     std::memset(rec, 0, sizeof(float) * BC_REC);
     std::memcpy(&rec[0], &known, 4);
     rec[1] = (float)u[0];

@@ -17,8 +17,21 @@ uint32_t known_mask(int q, uint32_t octants);
 // Build the 7x7 system A for known set `known`, relaxation omega and wall
 // velocity u (unscaled), in FP64, and invert it. Returns false if singular.
 // Unknown vector x = (rho, P_xx, P_xy, P_xz, P_yy, P_yz, P_zz), P = rho m;
-// right-hand side rho_I (1, m_I).
-bool build_bc_inverse(int q, uint32_t known, double omega, const double u[3], double ainv[49], double* cond = nullptr);
+// right-hand side rho_I (1, m_I). The sums of A run over the known
+// populations of lattice closure_q (see closure_lattice).
+bool build_bc_inverse(int q, uint32_t known, double omega, const double u[3], double ainv[49], double* cond,
+                      int closure_q);
+
+// Lattice whose weights and velocities form a boundary node's closure: the
+// lattice itself, except D3Q27 nodes on a single plane wall (octant byte
+// NORTH, SOUTH, WEST, EAST, FRONT or BACK), whose closure is the D3Q19 one
+// as in the reference's only D3Q27 wall formulas
+// (cases/parallelPlates_D3Q27/bc_definition.inc:18-72). Edges, corners and
+// voxel patterns, for which the reference has no D3Q27 formula, use the
+// consistent D3Q27 closure: with the D3Q19 one a D3Q27 duct does not
+// converge (steady duct error 1.3 % -> 1.9 % as the cross-section doubles,
+// against 0.84 % -> 0.20 % with the D3Q27 closure).
+int closure_lattice(int q, uint32_t octants);
 
 struct BcClassTable {
     std::vector<uint32_t> keys;      // node-type key (octants | kind | velocity index) per class, index 1..

@@ -92,15 +92,25 @@ bool solve7(double A[7][7], double b[7], double x[7]) {
 //   sum_{K} w_i [rho (1 + 3 c.u) + 4.5 H_i:((1-omega) P + omega rho u u)] = rho_I
 //   sum_{K} w_i H_i,ab [rho (1 + 3 c.u) + 4.5 H_i:P]                     = rho_I m_I,ab
 // with rho_I = sum_K f_i, rho_I m_I = sum_K f_i H_i.
-bool bc_solve(int q, uint32_t known, double omega, const double u[3], const double* pop, double out[10]) {
+// Lattice of the closure: D3Q19 coefficients for D3Q27 nodes on a single
+// plane wall, as the reference's only D3Q27 wall formulas have them
+// (cases/parallelPlates_D3Q27/bc_definition.inc:18-72); the lattice's own
+// otherwise (edges, corners and voxel patterns have no D3Q27 formula in the
+// reference, and the D3Q19 closure there keeps a D3Q27 duct from
+// converging). Octant bytes of the six plane walls: src/nodeTypeMap.h.
+int closure_q(int q, uint32_t octants) {
+    const bool face = octants == 0xCC || octants == 0x33 || octants == 0x55 || octants == 0xAA ||
+                      octants == 0xF0 || octants == 0x0F;
+    return (q == 27 && face) ? 19 : q;
+}
+
+bool bc_solve(int q, int qc, uint32_t known, double omega, const double u[3], const double* pop, double out[10]) {
     double rI = 0, S[6] = {0, 0, 0, 0, 0, 0};
     double A[7][7] = {};
     const double coef[6] = {1, 2, 2, 1, 2, 1};
     const double uu[6] = {u[0] * u[0], u[0] * u[1], u[0] * u[2], u[1] * u[1], u[1] * u[2], u[2] * u[2]};
-    // Incoming moments run over every known population of the lattice. The
-    // closure matrix is the D3Q19 one for either lattice: the reference's
-    // D3Q27 wall formulas (cases/parallelPlates_D3Q27/bc_definition.inc:18-72)
-    // pair D3Q27 incoming moments with the D3Q19 coefficients.
+    // Incoming moments run over every known population of the lattice; the
+    // closure matrix over the known populations of lattice qc (closure_q).
     for (int i = 0; i < q; i++) {
         if (!((known >> i) & 1u)) continue;
         double h[6];
@@ -108,11 +118,11 @@ bool bc_solve(int q, uint32_t known, double omega, const double u[3], const doub
         rI += pop[i];
         for (int k = 0; k < 6; k++) S[k] += pop[i] * h[k];
     }
-    for (int i = 0; i < 19; i++) {
+    for (int i = 0; i < qc; i++) {
         if (!((known >> i) & 1u)) continue;
         double h[6];
         herm(i, h);
-        const double w = W(19, i);
+        const double w = W(qc, i);
         const double cu = CX[i] * u[0] + CY[i] * u[1] + CZ[i] * u[2];
         double huu = 0;
         for (int k = 0; k < 6; k++) huu += coef[k] * h[k] * uu[k];
@@ -395,7 +405,7 @@ static void one_step(orc* o) {
                     const double zero[3] = {0, 0, 0};
                     const double* u = kind == 1 ? o->vel[vi] : zero;
                     const uint32_t known = known_mask_impl(q, t & 0xFFu);
-                    if (!bc_solve(q, known, o->omega, u, f, m)) {
+                    if (!bc_solve(q, closure_q(q, t & 0xFFu), known, o->omega, u, f, m)) {
                         for (int k = 0; k < 10; k++) m[k] = NaN;
                     }
                 }
@@ -451,7 +461,7 @@ int orc_get_threads(void) { return omp_get_max_threads(); }
 
 int orc_bc_node(int q, uint32_t t, double omega, const double u[3], const double* pop, double out[10]) {
     const uint32_t known = known_mask_impl(q, t & 0xFFu);
-    return bc_solve(q, known, omega, u, pop, out) ? 0 : -1;
+    return bc_solve(q, closure_q(q, t & 0xFFu), known, omega, u, pop, out) ? 0 : -1;
 }
 uint32_t orc_known_mask(int q, uint32_t t) { return known_mask_impl(q, t & 0xFFu); }
 void orc_reconstruct(int q, const double m[10], double* pop) { reconstruct(q, m, pop); }

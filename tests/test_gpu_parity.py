@@ -312,3 +312,49 @@ def test_reduced_precision_forced_channel(storage):
     err = np.linalg.norm(ux - ana) / np.linalg.norm(ana)
     say("storage", storage, "channel rel err", err)
     assert err < (2e-3 if storage == mrlbm.STORE_FP16 else 2e-2)
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_duct_start_up_vs_oracle_and_series(q):
+    """Force-driven rectangular duct from rest (the precision case's
+    geometry at 8 x 32 x 48): FP32 storage against the oracle to 1e-5, and
+    the streamed velocity against the transient duct series (the oracle
+    itself is within 1.7e-3 (D3Q19) / 2.1e-3 (D3Q27) of the series here,
+    lattice and time discretisation)."""
+    from mrlbm import analytic
+    shape, nu, g, steps = (8, 32, 48), 0.05, 1e-5, 3000
+    types = geometry.classify(shape, (0, 1, 1))
+    gm, o, orc = run_pair(shape, q, 0.5 + 3 * nu, steps, walls=(0, 1, 1), types=types, force=(g, 0, 0),
+                          init="uniform")
+    assert orc.count_nan() == 0
+    e = rel_l2(gm, o)
+    say("rel L2 vs oracle", e)
+    assert max(e.values()) <= TOL
+    ux = (gm[1] / 3).reshape(shape, order="F").mean(axis=0) - g  # stored u carries +F/rho
+    ser = analytic.duct_start_up(shape[1], shape[2], g, nu, t=steps)
+    err = np.linalg.norm(ux - ser) / np.linalg.norm(ser)
+    say("rel L2 vs series", err)
+    assert err < 2.5e-3
+
+
+@pytest.mark.parametrize("storage,tol", [(mrlbm.STORE_FP16, 1e-3), (mrlbm.STORE_BF16, 1e-2)])
+def test_reduced_precision_duct_d3q27(storage, tol):
+    """The D3Q27 duct with FP16 / BF16 moment storage against the FP64
+    oracle: relative L2 of the velocity field within tol."""
+    shape, nu, g, steps = (8, 32, 48), 0.05, 1e-5, 3000
+    types = geometry.classify(shape, (0, 1, 1))
+    orc = O.Oracle(*shape, q=27, tau=0.5 + 3 * nu, force=(g, 0, 0), walls=(0, 1, 1))
+    orc.set_node_types(types)
+    orc.init_uniform()
+    orc.step(steps)
+    o = orc.moments()
+    with mrlbm.Lattice(shape, q=27, tau=0.5 + 3 * nu, walls=(0, 1, 1), force=(g, 0, 0), storage=storage,
+                       tile=(8, 8, 8)) as lat:
+        lat.set_node_types(types)
+        lat.init_uniform()
+        lat.step(steps)
+        m = lat.moments()
+    eu = np.linalg.norm(m[1:4] - o[1:4]) / np.linalg.norm(o[1:4])
+    er = np.linalg.norm(m[0] - o[0]) / np.linalg.norm(o[0] + 1.0)  # rho - 1 stored; relative to rho
+    say("storage", storage, "u rel L2", eu, "rho rel L2", er)
+    assert eu < tol and er < tol

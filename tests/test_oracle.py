@@ -230,3 +230,41 @@ def test_rel_l2_helper_detects_difference():
     b = a.copy()
     b[1, 0] += 1e-3
     assert rel_l2(a, b)["u"] > 1e-3
+
+
+def test_duct_series_limits():
+    """The rectangular-duct series (mrlbm.analytic) against its own limits:
+    a very wide duct is plane Poiseuille in its middle; t -> 0 is rest; the
+    steady profile is reached for t >> 1/(nu lambda_11)."""
+    from mrlbm import analytic
+    u = analytic.duct_start_up(33, 2001, 1e-6, 0.05)
+    pl = analytic.channel_steady(33, 1e-6, 0.05)
+    assert np.abs(u[:, 1000] - pl).max() < 1e-4 * pl.max()
+    assert np.abs(analytic.duct_start_up(16, 24, 1e-6, 0.05, t=0.0)).max() == 0.0
+    steady = analytic.duct_start_up(16, 24, 1e-6, 0.05)
+    late = analytic.duct_start_up(16, 24, 1e-6, 0.05, t=1e6)
+    assert np.abs(late - steady).max() < 1e-12 * steady.max()
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_duct_flow_converges_to_series(q):
+    """Force-driven duct (x periodic, walls on y and z) run to steady state
+    against the series: the error must fall at least 3x as the cross-section
+    doubles (second order gives 4x) and be below 0.3 % at 32 x 48."""
+    from mrlbm import analytic, geometry
+    nu = 0.05
+    errs = []
+    for k in (1, 2):
+        shape = (4, 16 * k, 24 * k)
+        g = 1e-5 / (k * k)
+        steps = int(25 / (nu * np.pi ** 2 * (1 / (shape[1] - 1) ** 2 + 1 / (shape[2] - 1) ** 2)))
+        o = O.Oracle(*shape, q=q, tau=0.5 + 3 * nu, force=(g, 0, 0), walls=(0, 1, 1))
+        o.set_node_types(geometry.classify(shape, (0, 1, 1)))
+        o.init_uniform()
+        o.step(steps)
+        assert o.count_nan() == 0
+        ux = (o.moments()[1] / 3).reshape(shape, order="F").mean(axis=0) - g  # stored u carries +F/rho
+        ser = analytic.duct_start_up(shape[1], shape[2], g, nu)
+        errs.append(np.linalg.norm(ux - ser) / np.linalg.norm(ser))
+    print("q", q, "steady duct errors", errs, flush=True)
+    assert errs[1] < errs[0] / 3 and errs[1] < 3e-3
