@@ -92,7 +92,7 @@ int mrlbm_create(const mrlbm_params* pp, mrlbm_ctx** out) {
     if (bx == 0 && by == 0 && bz == 0) {
         // default: 8x8x8 tiles (the fastest measured shape), or for the
         // column kernel the longest segment that divides the lattice
-        const int cand[3][3] = {{8, 8, 8}, {16, 16, 64}, {16, 16, 32}};
+        const int cand[3][3] = {{8, 8, 8}, {16, 16, 32}, {16, 16, 64}};
         for (int c = 0; c < 3; c++) {
             const bool col = cand[c][0] * cand[c][1] == 256;
             if (kernel != MRLBM_KERNEL_COLUMN && col) continue;

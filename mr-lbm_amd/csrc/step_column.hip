@@ -109,6 +109,22 @@ __global__ __launch_bounds__(BX* BY) void k_step_column(StepArgs a) {
     uint32_t cls_next = 0;
     if constexpr (HAS_BC) cls_next = a.cls[cbase];
 
+    // x and y face populations entering plane k, loaded one plane ahead so
+    // that their latency overlaps the previous plane's work
+    constexpr int QF = VS<Q>::QF;
+    float px[QF], py[QF];
+    auto fetch_xy = [&](int k) {
+        if (tx == 0)
+            load_face<Q, BX, BY, BZ, 1>(a, tx, ty, k, bx, by, bz, px);
+        else if (tx == BX - 1)
+            load_face<Q, BX, BY, BZ, 0>(a, tx, ty, k, bx, by, bz, px);
+        if (ty == 0)
+            load_face<Q, BX, BY, BZ, 3>(a, tx, ty, k, bx, by, bz, py);
+        else if (ty == BY - 1)
+            load_face<Q, BX, BY, BZ, 2>(a, tx, ty, k, bx, by, bz, py);
+    };
+    fetch_xy(0);
+
     for (int k = 0; k < BZ; k++) {
         const uint32_t cls = cls_next;
         if constexpr (HAS_BC) {
@@ -123,13 +139,14 @@ __global__ __launch_bounds__(BX* BY) void k_step_column(StepArgs a) {
         // populations crossing the tile's x/y faces, then the segment's z
         // ends (X, Y, Z order: the later branch wins, as in popLoad.inc)
         if (tx == 0)
-            get_face<Q, BX, BY, BZ, 1>(a, tx, ty, k, bx, by, bz, in0);
+            commit_face<Q, 1>(px, in0);
         else if (tx == BX - 1)
-            get_face<Q, BX, BY, BZ, 0>(a, tx, ty, k, bx, by, bz, in0);
+            commit_face<Q, 0>(px, in0);
         if (ty == 0)
-            get_face<Q, BX, BY, BZ, 3>(a, tx, ty, k, bx, by, bz, in0);
+            commit_face<Q, 3>(py, in0);
         else if (ty == BY - 1)
-            get_face<Q, BX, BY, BZ, 2>(a, tx, ty, k, bx, by, bz, in0);
+            commit_face<Q, 2>(py, in0);
+        if (k + 1 < BZ) fetch_xy(k + 1);
         if (k == 0) get_face<Q, BX, BY, BZ, 5>(a, tx, ty, k, bx, by, bz, in0);
         if (k == BZ - 1) get_face<Q, BX, BY, BZ, 4>(a, tx, ty, k, bx, by, bz, in0);
 

@@ -156,11 +156,12 @@ __device__ __forceinline__ void put_face_c(float* __restrict__ g, int u, int v, 
     });
 }
 
-// Read the incoming populations through face FACE of this tile. FACE is the
-// neighbour's outgoing face: a node at tx==0 reads X_1 of tile bx-1, etc.
+// Read the incoming populations through face FACE of this tile, in slot
+// order (vals[s] is population face_pop(FACE, s)). FACE is the neighbour's
+// outgoing face: a node at tx==0 reads X_1 of tile bx-1, etc.
 template <int Q, int BX, int BY, int BZ, int FACE>
-__device__ __forceinline__ void get_face(const StepArgs& a, int tx, int ty, int tz, int bx, int by, int bz,
-                                         float* pop) {
+__device__ __forceinline__ void load_face(const StepArgs& a, int tx, int ty, int tz, int bx, int by, int bz,
+                                          float* vals) {
     constexpr int QF = VS<Q>::QF;
     const auto g = gptr(a.gin[FACE]);
     static_for<QF>([&](auto SI) {
@@ -178,8 +179,25 @@ __device__ __forceinline__ void get_face(const StepArgs& a, int tx, int ty, int 
             idx = gidx_y(ux_, uz_, s, st, BX, BZ, QF);
         else
             idx = gidx_z(ux_, uy_, s, st, BX, BY, QF);
-        pop[p] = g[idx];
+        vals[s] = g[idx];
     });
+}
+
+// File face values (load_face order) into the population array.
+template <int Q, int FACE>
+__device__ __forceinline__ void commit_face(const float* vals, float* pop) {
+    static_for<VS<Q>::QF>([&](auto SI) {
+        constexpr int s = decltype(SI)::value;
+        pop[face_pop(FACE, s)] = vals[s];
+    });
+}
+
+template <int Q, int BX, int BY, int BZ, int FACE>
+__device__ __forceinline__ void get_face(const StepArgs& a, int tx, int ty, int tz, int bx, int by, int bz,
+                                         float* pop) {
+    float vals[VS<Q>::QF];
+    load_face<Q, BX, BY, BZ, FACE>(a, tx, ty, tz, bx, by, bz, vals);
+    commit_face<Q, FACE>(vals, pop);
 }
 
 }  // namespace mrlbm
