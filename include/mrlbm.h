@@ -151,6 +151,27 @@ int mrlbm_write_moments(mrlbm_ctx* ctx, const double* in10n);
  * src/treatData.cu:105-165, src/reduction.cu:99-155). */
 int mrlbm_reduce(mrlbm_ctx* ctx, double* mean_mass, double* mean_ke);
 
+/* Diagnostics (device-side; replace the reference's treatData family,
+ * src/treatData.cu, which copies one node at a time to the host). */
+/* Stored-convention moments (10 per point, FP64) at n nodes given as x,y,z
+ * triples: probe points and lines. Replaces probeExport and
+ * velocityProfile (src/treatData.cu:501-584, :383-498). */
+int mrlbm_sample(mrlbm_ctx* ctx, int n, const int32_t* xyz, double* out10n);
+/* Mean over all nodes of each stored moment, FP64.
+ * Replaces mean_moment (src/treatData.cu:36-102). */
+int mrlbm_moment_means(mrlbm_ctx* ctx, double out[10]);
+/* Running mean flow: accumulate the current rho, ux, uy, uz of every node
+ * (FP64 sums, 32 B/node allocated on first use), reset, and read the mean
+ * (4 x N, x-fastest) with the sample count. Replaces the MEAN_FLOW
+ * accumulation of fMom_mean (src/main.cu, src/treatData.cu). */
+int mrlbm_mean_flow_accumulate(mrlbm_ctx* ctx);
+int mrlbm_mean_flow_reset(mrlbm_ctx* ctx);
+int mrlbm_mean_flow_read(mrlbm_ctx* ctx, double* out4n, int64_t* count);
+/* Turbulent kinetic energy of the current state against the running mean:
+ * mean over nodes of rho |u - mean u|^2 / 2.
+ * Replaces turbulentKineticEnergy (src/treatData.cu:236-296). */
+int mrlbm_turbulent_ke(mrlbm_ctx* ctx, double* tke);
+
 /* Checkpoint = moments + current ghost set + step + parameters, raw
  * little-endian, versioned header. Replaces saveSimCheckpoint /
  * loadSimCheckpoint (src/checkpoint.cu:372-382, :353-369). */

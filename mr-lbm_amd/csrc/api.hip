@@ -65,6 +65,7 @@ int mrlbm_destroy(mrlbm_ctx* c) {
     hipFree(c->d_cls);
     hipFree(c->d_bc);
     hipFree(c->d_tiles);
+    hipFree(c->d_mean);
     for (int s = 0; s < 2; s++)
         for (int f = 0; f < 6; f++) hipFree(c->d_ghost[s][f]);
     hipFree(c->d_partial);

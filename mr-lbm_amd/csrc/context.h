@@ -54,6 +54,8 @@ struct mrlbm_ctx {
     uint64_t tile_flags_version = ~0ull;
     uint32_t* d_tiles = nullptr;      // all-bulk tiles, then the others (tile kernel with boundaries)
     int n_bulk_tiles = 0;
+    double* d_mean = nullptr;         // running sums of rho, ux, uy, uz (4 x N, x-fastest), allocated on first use
+    int64_t mean_count = 0;
 
     size_t mom_index(int x, int y, int z, int m) const {
         const int tile = (x / bx) + ntx * ((y / by) + nty * (z / bz));

@@ -137,6 +137,40 @@ class Lattice:
         _lib.check(_lib.load().mrlbm_reduce(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    # -- diagnostics ----------------------------------------------------
+    def sample(self, points):
+        """Stored-convention moments (n, 10) at integer nodes points (n, 3)."""
+        p = np.ascontiguousarray(points, dtype=np.int32).reshape(-1, 3)
+        out = np.empty((len(p), 10))
+        if len(p):
+            _lib.check(_lib.load().mrlbm_sample(self._h, len(p), p.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                out.ctypes.data_as(_lib._dp)))
+        return out
+
+    def moment_means(self):
+        out = np.empty(10)
+        _lib.check(_lib.load().mrlbm_moment_means(self._h, out.ctypes.data_as(_lib._dp)))
+        return out
+
+    def mean_flow_accumulate(self):
+        _lib.check(_lib.load().mrlbm_mean_flow_accumulate(self._h))
+
+    def mean_flow_reset(self):
+        _lib.check(_lib.load().mrlbm_mean_flow_reset(self._h))
+
+    def mean_flow(self):
+        """(rho, ux, uy, uz) time means as (nx, ny, nz) arrays, and the count."""
+        out = np.empty(4 * self.n)
+        cnt = ctypes.c_int64()
+        _lib.check(_lib.load().mrlbm_mean_flow_read(self._h, out.ctypes.data_as(_lib._dp), ctypes.byref(cnt)))
+        f = out.reshape(4, self.n)
+        return tuple(v.reshape(self.shape, order="F") for v in f), cnt.value
+
+    def turbulent_ke(self):
+        v = ctypes.c_double()
+        _lib.check(_lib.load().mrlbm_turbulent_ke(self._h, ctypes.byref(v)))
+        return v.value
+
     def checkpoint_save(self, path):
         _lib.check(_lib.load().mrlbm_checkpoint_save(self._h, str(path).encode()))
 

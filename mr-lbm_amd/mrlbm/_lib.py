@@ -80,6 +80,12 @@ _SIGS = {
     "mrlbm_create": [ctypes.POINTER(Params), ctypes.POINTER(_vp)],
     "mrlbm_destroy": [_vp],
     "mrlbm_set_bc_velocity": [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+    "mrlbm_sample": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int32), _dp],
+    "mrlbm_moment_means": [_vp, _dp],
+    "mrlbm_mean_flow_accumulate": [_vp],
+    "mrlbm_mean_flow_reset": [_vp],
+    "mrlbm_mean_flow_read": [_vp, _dp, ctypes.POINTER(ctypes.c_int64)],
+    "mrlbm_turbulent_ke": [_vp, _dp],
     "mrlbm_set_node_types": [_vp, _u32p],
     "mrlbm_set_geometry": [_vp, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int, _dp],
     "mrlbm_init_fields": [_vp, _dp, _dp, _dp, _dp, ctypes.c_int],
