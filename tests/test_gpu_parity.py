@@ -358,3 +358,17 @@ def test_reduced_precision_duct_d3q27(storage, tol):
     er = np.linalg.norm(m[0] - o[0]) / np.linalg.norm(o[0] + 1.0)  # rho - 1 stored; relative to rho
     say("storage", storage, "u rel L2", eu, "rho rel L2", er)
     assert eu < tol and er < tol
+
+
+def test_lid_driven_cavity_64_re100_1000_steps():
+    """The plain-method CPU case at its full size: D3Q19 cavity 64^3, lid
+    u = 0.05 on the top z plane, Re = 100 (tau = 0.5 + 3 u N / Re), rest
+    start, 1000 steps; GPU FP32 against the FP64 oracle."""
+    n, u = 64, 0.05
+    types = geometry.cavity(n, lid_axis=2)
+    tau = 0.5 + 3 * u * n / 100.0
+    g, o, orc = run_pair((n, n, n), 19, tau, 1000, walls=(1, 1, 1), types=types, init="uniform", bc_vel=(u, 0, 0))
+    assert orc.count_nan() == 0
+    e = rel_l2(g, o)
+    say("rel L2", e)
+    assert max(e.values()) <= TOL
