@@ -295,6 +295,13 @@ typedef struct {
  * U[3, Ny/6]; gx = 10^U[-7,-5]; nu = U[0.01,0.1]; steps = U{500..5000}). */
 int mrlbm_generate_duct_requests(uint64_t seed, int n, double rate, mrlbm_request* out);
 int mrlbm_serve_run(const mrlbm_serve_params* params, mrlbm_request* reqs, int nreq, mrlbm_serve_stats* stats);
+/* The same serving loop on a simulated device (no GPU needed): a virtual
+ * clock, rounds of q steps over N batched nodes taking N q / rate_mlups us,
+ * each admission admit_s seconds. Results (mass, moments) are not computed;
+ * start/finish/latency and the statistics are. For testing and sizing the
+ * scheduler. */
+int mrlbm_serve_simulate(const mrlbm_serve_params* params, mrlbm_request* reqs, int nreq, double rate_mlups,
+                         double admit_s, mrlbm_serve_stats* stats);
 
 /* Measurement utility (no reference counterpart): time `iters` plain
  * device-to-device copies of `bytes` on `device`; *ms_per_copy = mean event

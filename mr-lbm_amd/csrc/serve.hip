@@ -7,7 +7,8 @@
 // request is admitted when the HBM budget and the batch-size limit allow:
 // its lattice is allocated, classified on the device (duct walls on y and
 // z, voxelised spheres) and initialised at rest on its own stream while the
-// current round runs, and it joins the batch at the next round boundary.
+// current round runs (or, if it arrived during the round, right after it),
+// and it joins the batch at the next round boundary.
 // All active lattices advance together, one launch per step, in rounds of
 // at most `quantum` steps (hipGraph-captured when a full quantum fits); a
 // round never passes a member's step budget, so members leave exactly at
@@ -51,9 +52,267 @@ double state_bytes(const mrlbm_request& r, int q, int storage, int bx, int by, i
 
 struct Active {
     int idx;
-    mrlbm_ctx* ctx;
+    mrlbm_ctx* ctx;  // null on the simulated device
     int remaining;
     double bytes;
+    double nodes;
+};
+
+// The serving loop, shared by the GPU and a simulated device. Dev supplies:
+//   double now();                           seconds since the loop started
+//   int admit(mrlbm_request&, Active&);     allocate, classify, initialise
+//   int add(Active&), remove(Active&);      batch membership
+//   int launch(int q, bool graph);          start a round of q steps
+//   int wait_round(double* busy_s);         end of the round in flight
+//   int finish(Active&, mrlbm_request&);    read results (mass, moments)
+//   void release(Active&);                  free its memory
+//   void idle(double seconds);              nothing to run until then
+template <typename Dev>
+int serve_loop(Dev& dev, const mrlbm_serve_params* sp, mrlbm_request* reqs, int nreq, mrlbm_serve_stats* st) {
+    const int bx = sp->bx ? sp->bx : 8, by = sp->by ? sp->by : 8, bz = sp->bz ? sp->bz : 8;
+    const int quantum = sp->quantum_steps > 0 ? sp->quantum_steps : 100;
+    const int graph = sp->graph_steps > 0 && quantum % sp->graph_steps == 0 ? sp->graph_steps : 0;
+    const int max_batch = sp->max_batch > 0 ? std::min(sp->max_batch, 4095) : 4095;
+    std::memset(st, 0, sizeof *st);
+    for (int i = 0; i < nreq; i++) {
+        if (reqs[i].nx <= 0 || reqs[i].ny <= 0 || reqs[i].nz <= 0 || reqs[i].nx % bx || reqs[i].ny % by ||
+            reqs[i].nz % bz)
+            return fail(MRLBM_EINVAL, "request %d extents are not positive multiples of the tile", i);
+        if (reqs[i].nspheres < 0 || reqs[i].nspheres > 8)
+            return fail(MRLBM_EINVAL, "request %d has %d spheres (0..8)", i, reqs[i].nspheres);
+        if (reqs[i].steps <= 0) return fail(MRLBM_EINVAL, "request %d has no steps", i);
+        if (state_bytes(reqs[i], sp->q, sp->storage, bx, by, bz) > sp->hbm_budget_bytes)
+            return fail(MRLBM_ENOMEM, "request %d alone exceeds the HBM budget", i);
+        if (i && reqs[i].arrival_s < reqs[i - 1].arrival_s)
+            return fail(MRLBM_EINVAL, "requests must be ordered by arrival time");
+        reqs[i].status = 1;  // pending
+    }
+    // ready: admitted and initialised, joins at the next round boundary;
+    // active: batch members; leaving: out of the batch, results being read
+    std::vector<Active> ready, active, leaving;
+    double used = 0, peak = 0, busy = 0, lu = 0;
+    int next = 0, done = 0, round_q = 0, rc = 0;
+    bool in_flight = false;
+
+    auto release_all = [&]() {
+        for (auto* v : {&ready, &active, &leaving})
+            for (auto& a : *v) dev.release(a);
+    };
+    auto fail_all = [&](int code) {
+        release_all();
+        return code;
+    };
+    // Admit visible requests (arrival order) while the budget and the batch
+    // limit allow; runs while a round is in flight.
+    auto admit = [&]() -> int {
+        const double tnow = dev.now();
+        while (next < nreq && (!sp->realtime || reqs[next].arrival_s <= tnow) &&
+               (int)(active.size() + ready.size()) < max_batch) {
+            mrlbm_request& r = reqs[next];
+            const double bytes = state_bytes(r, sp->q, sp->storage, bx, by, bz);
+            if (used + bytes > sp->hbm_budget_bytes) break;
+            Active a{next, nullptr, r.steps, bytes, (double)r.nx * r.ny * r.nz};
+            const int e = dev.admit(r, a);
+            if (e) return e;
+            r.start_s = dev.now();
+            r.status = 2;  // running
+            ready.push_back(a);
+            used += bytes;
+            peak = std::max(peak, used);
+            next++;
+        }
+        return MRLBM_OK;
+    };
+    auto finish = [&]() -> int {
+        for (auto& a : leaving) {
+            mrlbm_request& r = reqs[a.idx];
+            const int e = dev.finish(a, r);
+            if (e) return e;
+            r.finish_s = dev.now();
+            r.latency_s = r.finish_s - (sp->realtime ? r.arrival_s : 0.0);
+            r.status = 0;
+            done++;
+        }
+        return MRLBM_OK;
+    };
+
+    while (done < nreq) {
+        if ((rc = admit())) return fail_all(rc);
+        if ((rc = finish())) return fail_all(rc);
+        if (in_flight) {
+            double b = 0;
+            if ((rc = dev.wait_round(&b))) return fail_all(rc);
+            busy += b;
+            st->rounds++;
+            in_flight = false;
+            for (auto& a : active) {
+                a.remaining -= round_q;
+                lu += (double)round_q * a.nodes;
+            }
+            // requests that arrived during the round join the next one
+            // (the first pass ran as the round started)
+            if ((rc = admit())) return fail_all(rc);
+        }
+        // memory of finished requests is released while the device is idle
+        for (auto& a : leaving) {
+            dev.release(a);
+            used -= a.bytes;
+        }
+        leaving.clear();
+        for (size_t i = 0; i < active.size();) {
+            if (active[i].remaining > 0) {
+                i++;
+                continue;
+            }
+            if ((rc = dev.remove(active[i]))) return fail_all(rc);
+            leaving.push_back(active[i]);
+            active.erase(active.begin() + i);
+        }
+        for (auto& a : ready) {
+            if ((rc = dev.add(a))) return fail_all(rc);
+            active.push_back(a);
+        }
+        ready.clear();
+        if (active.empty()) {
+            if (!leaving.empty()) continue;  // read their results now
+            if ((rc = admit())) return fail_all(rc);
+            if (ready.empty() && next < nreq && sp->realtime) {
+                const double wait = reqs[next].arrival_s - dev.now();
+                if (wait > 0) dev.idle(wait);
+            }
+            continue;
+        }
+        // next round: at most `quantum` steps, never past a member's budget
+        int q = quantum;
+        for (auto& a : active) q = std::min(q, a.remaining);
+        if ((rc = dev.launch(q, graph && q == quantum))) return fail_all(rc);
+        round_q = q;
+        in_flight = true;
+    }
+    release_all();
+    const double wall = dev.now();
+    std::vector<double> lat;
+    for (int i = 0; i < nreq; i++) lat.push_back(reqs[i].latency_s);
+    std::sort(lat.begin(), lat.end());
+    auto pct = [&](double p) {
+        if (lat.empty()) return 0.0;
+        const double pos = p * (lat.size() - 1);
+        const size_t i = (size_t)pos;
+        const double f = pos - i;
+        return i + 1 < lat.size() ? lat[i] * (1 - f) + lat[i + 1] * f : lat[i];
+    };
+    st->wall_s = wall;
+    st->busy_s = busy;
+    st->lattice_updates = lu;
+    st->mlups = wall > 0 ? lu / wall / 1e6 : 0;
+    st->mlups_busy = busy > 0 ? lu / busy / 1e6 : 0;
+    st->p50_s = pct(0.50);
+    st->p99_s = pct(0.99);
+    st->max_s = lat.empty() ? 0 : lat.back();
+    st->peak_bytes = peak;
+    return MRLBM_OK;
+}
+
+// The GPU: one context per request in one batch, wall clock.
+struct GpuDevice {
+    const mrlbm_serve_params* sp;
+    int bx, by, bz;
+    mrlbm_batch* batch = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+    double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+    int admit(mrlbm_request& r, Active& a) {
+        mrlbm_params p{};
+        p.nx = r.nx;
+        p.ny = r.ny;
+        p.nz = r.nz;
+        p.q = sp->q;
+        p.bx = bx;
+        p.by = by;
+        p.bz = bz;
+        p.storage = sp->storage;
+        p.wall_y = p.wall_z = 1;
+        p.kernel = MRLBM_KERNEL_TILE;
+        p.tau = 0.5 + 3.0 * r.nu;
+        p.rho0 = 1.0;
+        p.fx = r.gx;
+        p.device = sp->device;
+        p.seed = r.seed;
+        mrlbm_ctx* c = nullptr;
+        int e = mrlbm_create(&p, &c);
+        if (!e) e = mrlbm_set_geometry(c, nullptr, r.nspheres, &r.sphere[0][0]);
+        if (!e) e = mrlbm_init_uniform(c, 1.0, 0.0, 0.0, 0.0, MRLBM_INIT_EQUILIBRIUM);
+        if (!e) e = mrlbm_reduce(c, &r.mass0, &r.ke0);
+        if (e) {
+            mrlbm_destroy(c);
+            return e;
+        }
+        a.ctx = c;
+        return MRLBM_OK;
+    }
+    int add(Active& a) { return mrlbm_batch_add(batch, a.ctx); }
+    int remove(Active& a) { return mrlbm_batch_remove(batch, a.ctx); }
+    int launch(int q, bool use_graph) {
+        if (hipEventRecord(ev0, batch_stream(batch)) != hipSuccess) return fail(MRLBM_EHIP, "event record");
+        const int e = mrlbm_batch_step(batch, q, use_graph ? sp->graph_steps : 0);
+        if (e) return e;
+        if (hipEventRecord(ev1, batch_stream(batch)) != hipSuccess) return fail(MRLBM_EHIP, "event record");
+        return MRLBM_OK;
+    }
+    int wait_round(double* busy_s) {
+        if (hipEventSynchronize(ev1) != hipSuccess) return fail(MRLBM_EHIP, "round failed");
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev0, ev1);
+        *busy_s = ms * 1e-3;
+        return MRLBM_OK;
+    }
+    int finish(Active& a, mrlbm_request& r) {
+        int e = mrlbm_reduce(a.ctx, &r.mass1, &r.ke1);
+        if (!e && r.out_moments) e = mrlbm_read_moments(a.ctx, r.out_moments);
+        return e;
+    }
+    void release(Active& a) {
+        mrlbm_destroy(a.ctx);
+        a.ctx = nullptr;
+    }
+    void idle(double seconds) { std::this_thread::sleep_for(std::chrono::duration<double>(std::min(seconds, 0.01))); }
+};
+
+// A simulated device for testing the loop without a GPU: a virtual clock;
+// a round of q steps over nodes N takes N q / rate seconds, admission
+// admit_s seconds of host time; results are not computed.
+struct SimDevice {
+    double rate_lups, admit_s;
+    double clock = 0, round_end = 0, round_len = 0;
+    double batch_nodes = 0;
+    double now() const { return clock; }
+    int admit(mrlbm_request& r, Active&) {
+        clock += admit_s;
+        r.mass0 = r.mass1 = r.ke0 = r.ke1 = 0;
+        return MRLBM_OK;
+    }
+    int add(Active& a) {
+        batch_nodes += a.nodes;
+        return MRLBM_OK;
+    }
+    int remove(Active& a) {
+        batch_nodes -= a.nodes;
+        return MRLBM_OK;
+    }
+    int launch(int q, bool) {
+        round_len = batch_nodes * q / rate_lups;
+        round_end = clock + round_len;
+        return MRLBM_OK;
+    }
+    int wait_round(double* busy_s) {
+        clock = std::max(clock, round_end);
+        *busy_s = round_len;
+        return MRLBM_OK;
+    }
+    int finish(Active&, mrlbm_request&) { return MRLBM_OK; }
+    void release(Active&) {}
+    void idle(double seconds) { clock += seconds; }
 };
 
 }  // namespace
@@ -93,187 +352,31 @@ int mrlbm_generate_duct_requests(uint64_t seed, int n, double rate, mrlbm_reques
 
 int mrlbm_serve_run(const mrlbm_serve_params* sp, mrlbm_request* reqs, int nreq, mrlbm_serve_stats* st) {
     if (!sp || (!reqs && nreq) || !st || nreq < 0) return fail(MRLBM_EINVAL, "null argument");
-    const int bx = sp->bx ? sp->bx : 8, by = sp->by ? sp->by : 8, bz = sp->bz ? sp->bz : 8;
-    const int quantum = sp->quantum_steps > 0 ? sp->quantum_steps : 100;
-    const int graph = sp->graph_steps > 0 && quantum % sp->graph_steps == 0 ? sp->graph_steps : 0;
-    const int max_batch = sp->max_batch > 0 ? std::min(sp->max_batch, 4095) : 4095;
-    std::memset(st, 0, sizeof *st);
-    for (int i = 0; i < nreq; i++) {
-        if (reqs[i].nx % bx || reqs[i].ny % by || reqs[i].nz % bz)
-            return fail(MRLBM_EINVAL, "request %d extents are not multiples of the tile", i);
-        if (reqs[i].nspheres < 0 || reqs[i].nspheres > 8)
-            return fail(MRLBM_EINVAL, "request %d has %d spheres (0..8)", i, reqs[i].nspheres);
-        if (state_bytes(reqs[i], sp->q, sp->storage, bx, by, bz) > sp->hbm_budget_bytes)
-            return fail(MRLBM_ENOMEM, "request %d alone exceeds the HBM budget", i);
-        if (i && reqs[i].arrival_s < reqs[i - 1].arrival_s)
-            return fail(MRLBM_EINVAL, "requests must be ordered by arrival time");
-        reqs[i].status = 1;  // pending
-    }
-    mrlbm_batch* batch = nullptr;
-    int rc = mrlbm_batch_create(sp->device, sp->q, bx, by, bz, sp->storage, &batch);
+    GpuDevice dev;
+    dev.sp = sp;
+    dev.bx = sp->bx ? sp->bx : 8;
+    dev.by = sp->by ? sp->by : 8;
+    dev.bz = sp->bz ? sp->bz : 8;
+    int rc = mrlbm_batch_create(sp->device, sp->q, dev.bx, dev.by, dev.bz, sp->storage, &dev.batch);
     if (rc) return rc;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess) {
-        mrlbm_batch_destroy(batch);
+    if (hipEventCreate(&dev.ev0) != hipSuccess || hipEventCreate(&dev.ev1) != hipSuccess) {
+        mrlbm_batch_destroy(dev.batch);
         return fail(MRLBM_EHIP, "cannot create events");
     }
+    dev.t0 = std::chrono::steady_clock::now();
+    rc = serve_loop(dev, sp, reqs, nreq, st);
+    mrlbm_batch_destroy(dev.batch);
+    (void)hipEventDestroy(dev.ev0);
+    (void)hipEventDestroy(dev.ev1);
+    return rc;
+}
 
-    using clk = std::chrono::steady_clock;
-    const auto t0 = clk::now();
-    auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
-    // ready: admitted and initialised, joins at the next round boundary;
-    // active: batch members; leaving: out of the batch, results being read
-    std::vector<Active> ready, active, leaving;
-    double used = 0, peak = 0, busy = 0, lu = 0;
-    int next = 0, done = 0, round_q = 0;
-    bool in_flight = false;
-
-    auto release = [&]() {
-        for (auto* v : {&ready, &active, &leaving})
-            for (auto& a : *v) mrlbm_destroy(a.ctx);
-        mrlbm_batch_destroy(batch);
-        (void)hipEventDestroy(ev0);
-        (void)hipEventDestroy(ev1);
-    };
-    auto fail_all = [&](int code) {
-        release();
-        return code;
-    };
-    // Admit visible requests while the budget and the batch limit allow;
-    // runs while a round is in flight (its work goes to each lattice's own
-    // stream), so set-up overlaps stepping.
-    auto admit = [&]() -> int {
-        const double tnow = now();
-        while (next < nreq && (!sp->realtime || reqs[next].arrival_s <= tnow) &&
-               (int)(active.size() + ready.size()) < max_batch) {
-            mrlbm_request& r = reqs[next];
-            const double bytes = state_bytes(r, sp->q, sp->storage, bx, by, bz);
-            if (used + bytes > sp->hbm_budget_bytes) break;
-            mrlbm_params p{};
-            p.nx = r.nx;
-            p.ny = r.ny;
-            p.nz = r.nz;
-            p.q = sp->q;
-            p.bx = bx;
-            p.by = by;
-            p.bz = bz;
-            p.storage = sp->storage;
-            p.wall_y = p.wall_z = 1;
-            p.kernel = MRLBM_KERNEL_TILE;
-            p.tau = 0.5 + 3.0 * r.nu;
-            p.rho0 = 1.0;
-            p.fx = r.gx;
-            p.device = sp->device;
-            p.seed = r.seed;
-            mrlbm_ctx* c = nullptr;
-            int e = mrlbm_create(&p, &c);
-            if (!e) e = mrlbm_set_geometry(c, nullptr, r.nspheres, &r.sphere[0][0]);
-            if (!e) e = mrlbm_init_uniform(c, 1.0, 0.0, 0.0, 0.0, MRLBM_INIT_EQUILIBRIUM);
-            if (!e) e = mrlbm_reduce(c, &r.mass0, &r.ke0);
-            if (e) {
-                mrlbm_destroy(c);
-                return e;
-            }
-            r.start_s = now();
-            r.status = 2;  // running
-            ready.push_back({next, c, r.steps, bytes});
-            used += bytes;
-            peak = std::max(peak, used);
-            next++;
-        }
-        return MRLBM_OK;
-    };
-    // Results of lattices that left the batch: FP64 mass, optional moments.
-    auto finish = [&]() -> int {
-        for (auto& a : leaving) {
-            mrlbm_request& r = reqs[a.idx];
-            int e = mrlbm_reduce(a.ctx, &r.mass1, &r.ke1);
-            if (!e && r.out_moments) e = mrlbm_read_moments(a.ctx, r.out_moments);
-            if (e) return e;
-            r.finish_s = now();
-            r.latency_s = r.finish_s - (sp->realtime ? r.arrival_s : 0.0);
-            r.status = 0;
-            done++;
-        }
-        return MRLBM_OK;
-    };
-
-    while (done < nreq) {
-        if ((rc = admit())) return fail_all(rc);
-        if ((rc = finish())) return fail_all(rc);
-        if (in_flight) {
-            // end of the round in flight
-            if (hipEventSynchronize(ev1) != hipSuccess) return fail_all(fail(MRLBM_EHIP, "round failed"));
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, ev0, ev1);
-            busy += ms * 1e-3;
-            st->rounds++;
-            in_flight = false;
-            for (auto& a : active) {
-                a.remaining -= round_q;
-                lu += (double)round_q * a.ctx->N;
-            }
-        }
-        // memory of finished requests is released while the GPU is idle
-        for (auto& a : leaving) {
-            mrlbm_destroy(a.ctx);
-            used -= a.bytes;
-        }
-        leaving.clear();
-        for (size_t i = 0; i < active.size();) {
-            if (active[i].remaining > 0) {
-                i++;
-                continue;
-            }
-            if ((rc = mrlbm_batch_remove(batch, active[i].ctx))) return fail_all(rc);
-            leaving.push_back(active[i]);
-            active.erase(active.begin() + i);
-        }
-        for (auto& a : ready) {
-            if ((rc = mrlbm_batch_add(batch, a.ctx))) return fail_all(rc);
-            active.push_back(a);
-        }
-        ready.clear();
-        if (active.empty()) {
-            if (!leaving.empty()) continue;  // read their results now
-            if ((rc = admit())) return fail_all(rc);
-            if (ready.empty() && next < nreq && sp->realtime) {
-                const double wait = reqs[next].arrival_s - now();
-                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
-            }
-            continue;
-        }
-        // next round: at most `quantum` steps, never past a member's budget
-        int q = quantum;
-        for (auto& a : active) q = std::min(q, a.remaining);
-        if (hipEventRecord(ev0, batch_stream(batch)) != hipSuccess) return fail_all(fail(MRLBM_EHIP, "event record"));
-        if ((rc = mrlbm_batch_step(batch, q, (graph && q == quantum) ? graph : 0))) return fail_all(rc);
-        if (hipEventRecord(ev1, batch_stream(batch)) != hipSuccess) return fail_all(fail(MRLBM_EHIP, "event record"));
-        round_q = q;
-        in_flight = true;
-    }
-    release();
-    const double wall = now();
-    std::vector<double> lat;
-    for (int i = 0; i < nreq; i++) lat.push_back(reqs[i].latency_s);
-    std::sort(lat.begin(), lat.end());
-    auto pct = [&](double p) {
-        if (lat.empty()) return 0.0;
-        const double pos = p * (lat.size() - 1);
-        const size_t i = (size_t)pos;
-        const double f = pos - i;
-        return i + 1 < lat.size() ? lat[i] * (1 - f) + lat[i + 1] * f : lat[i];
-    };
-    st->wall_s = wall;
-    st->busy_s = busy;
-    st->lattice_updates = lu;
-    st->mlups = lu / wall / 1e6;
-    st->mlups_busy = busy > 0 ? lu / busy / 1e6 : 0;
-    st->p50_s = pct(0.50);
-    st->p99_s = pct(0.99);
-    st->max_s = lat.empty() ? 0 : lat.back();
-    st->peak_bytes = peak;
-    return MRLBM_OK;
+int mrlbm_serve_simulate(const mrlbm_serve_params* sp, mrlbm_request* reqs, int nreq, double rate_mlups,
+                         double admit_s, mrlbm_serve_stats* st) {
+    if (!sp || (!reqs && nreq) || !st || nreq < 0 || !(rate_mlups > 0) || !(admit_s >= 0))
+        return fail(MRLBM_EINVAL, "bad argument");
+    SimDevice dev{rate_mlups * 1e6, admit_s};
+    return serve_loop(dev, sp, reqs, nreq, st);
 }
 
 }  // extern "C"

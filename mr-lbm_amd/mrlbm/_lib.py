@@ -120,6 +120,8 @@ _SIGS = {
     "mrlbm_voxelize_spheres": [ctypes.c_int] * 4 + [_dp, ctypes.POINTER(ctypes.c_uint8)],
     "mrlbm_read_voxel_csv": [ctypes.c_char_p] + [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_uint8)],
     "mrlbm_generate_duct_requests": [ctypes.c_uint64, ctypes.c_int, ctypes.c_double, ctypes.POINTER(Request)],
+    "mrlbm_serve_simulate": [ctypes.POINTER(ServeParams), ctypes.POINTER(Request), ctypes.c_int, ctypes.c_double,
+                             ctypes.c_double, ctypes.POINTER(ServeStats)],
     "mrlbm_serve_run": [ctypes.POINTER(ServeParams), ctypes.POINTER(Request), ctypes.c_int,
                         ctypes.POINTER(ServeStats)],
 }

@@ -25,6 +25,19 @@ def run(requests, storage=0, q=19, tile=(8, 8, 8), budget_gb=200.0, quantum=100,
     return {k: getattr(st, k) for k, _ in _lib.ServeStats._fields_}
 
 
+def simulate(requests, rate_mlups, admit_s=0.0, storage=0, q=19, tile=(8, 8, 8), budget_gb=200.0, quantum=100,
+             graph_steps=50, max_batch=0, realtime=True):
+    """The same loop on a simulated device (no GPU): virtual clock, rounds
+    taking nodes * steps / rate, admissions admit_s each. Returns the stats
+    dict; start/finish/latency are written into the array."""
+    p = _lib.ServeParams(0, q, storage, tile[0], tile[1], tile[2], budget_gb * 1e9, quantum, graph_steps,
+                         max_batch, 1 if realtime else 0)
+    st = _lib.ServeStats()
+    _lib.check(_lib.load().mrlbm_serve_simulate(ctypes.byref(p), requests, len(requests), float(rate_mlups),
+                                                float(admit_s), ctypes.byref(st)))
+    return {k: getattr(st, k) for k, _ in _lib.ServeStats._fields_}
+
+
 def attach_output(req):
     """Allocate a moments buffer for one request; returns the numpy array."""
     n = req.nx * req.ny * req.nz
