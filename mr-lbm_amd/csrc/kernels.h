@@ -10,7 +10,7 @@
 namespace mrlbm {
 
 // Storage codecs. Arithmetic is always FP32; a codec maps a stored value
-// to/from float. Reduced-precision codecs store v * scale (a power of two,
+// to/from float. Loads are non-temporal: moments are streamed once per step. Reduced-precision codecs store v * scale (a power of two,
 // so the scaling is exact) and round stochastically: `rnd` carries uniform
 // random bits (one draw per value, counter-based, see sr_bits below); adding
 // them below the kept mantissa bits and truncating gives an unbiased
@@ -19,7 +19,7 @@ template <int S> struct Codec;
 template <> struct Codec<0> {
     using T = float;
     template <typename P>
-    __device__ __forceinline__ static float ld(P p, size_t i, float) { return p[i]; }
+    __device__ __forceinline__ static float ld(P p, size_t i, float) { return __builtin_nontemporal_load(&p[i]); }
     __device__ __forceinline__ static T enc(float v, float, uint32_t) { return v; }
     __device__ __forceinline__ static float dec(T b, float) { return b; }
 };
@@ -34,7 +34,7 @@ template <> struct Codec<1> {  // IEEE binary16, 13 dropped bits rounded stochas
         return __builtin_bit_cast(T, (_Float16)t);  // exact for normal binary16 values
     }
     template <typename P>
-    __device__ __forceinline__ static float ld(P p, size_t i, float rscale) { return dec(p[i], rscale); }
+    __device__ __forceinline__ static float ld(P p, size_t i, float rscale) { return dec(__builtin_nontemporal_load(&p[i]), rscale); }
 };
 template <> struct Codec<2> {  // bfloat16, 16 dropped bits rounded stochastically
     using T = uint16_t;
@@ -44,7 +44,7 @@ template <> struct Codec<2> {  // bfloat16, 16 dropped bits rounded stochastical
         return (T)((u + (rnd & 0xFFFFu)) >> 16);
     }
     template <typename P>
-    __device__ __forceinline__ static float ld(P p, size_t i, float rscale) { return dec(p[i], rscale); }
+    __device__ __forceinline__ static float ld(P p, size_t i, float rscale) { return dec(__builtin_nontemporal_load(&p[i]), rscale); }
 };
 
 // Counter-based random bits for stochastic rounding: a 32-bit mix of the

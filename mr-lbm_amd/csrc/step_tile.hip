@@ -168,7 +168,10 @@ __device__ __forceinline__ void tile_fetch(const StepArgs& a, const float* const
     const auto mom = gptr((const typename Codec<S>::T*)a.mom);
     const size_t mb = (size_t)tile * 10 * T + t;
 #pragma unroll
-    for (int k = 0; k < 10; k++) in.m[k] = mom[mb + (size_t)k * T];
+    // moments are read and written exactly once per step: non-temporal
+    // accesses keep them from displacing the ghost-plane lines that
+    // neighbouring tiles read (measured +8 % on 256^3 FP32)
+    for (int k = 0; k < 10; k++) in.m[k] = __builtin_nontemporal_load(&mom[mb + (size_t)k * T]);
     // wave W owns incoming chunks W, W + NW, ...; every wave fills all
     // RMAX slots (zeros past its last chunk) so that no slot index depends
     // on the wave (a wave-dependent index would move the array to scratch)

@@ -135,7 +135,7 @@ __device__ __forceinline__ void store_node(const StepArgs& a, size_t mb, const t
     constexpr size_t T = (size_t)BX * BY * BZ;
     const auto mom = gptr((typename Codec<S>::T*)a.mom);
 #pragma unroll
-    for (int k = 0; k < 10; k++) mom[mb + k * T] = q[k];
+    for (int k = 0; k < 10; k++) __builtin_nontemporal_store(q[k], &mom[mb + k * T]);  // streamed once per step
 }
 
 // Write the outgoing populations of face FACE of node (u, v) of the face.
