@@ -206,6 +206,7 @@ int mrlbm_set_node_types(mrlbm_ctx* c, const uint32_t* types) {
     hipFree(c->d_bc);
     c->d_cls = nullptr;
     c->d_bc = nullptr;
+    c->bc_bytes = 0;
     c->has_bc = tab.has_bc;
     c->has_solid = tab.has_solid;
     c->n_classes = (int)tab.keys.size() - 1;
@@ -219,6 +220,7 @@ int mrlbm_set_node_types(mrlbm_ctx* c, const uint32_t* types) {
             for (int x = 0; x < nx; x++) blk[c->node_index(x, y, z)] = tab.class_of[x + (size_t)nx * (y + (size_t)ny * z)];
     HIPCHK(hipMalloc((void**)&c->d_cls, c->N));
     HIPCHK(hipMalloc((void**)&c->d_bc, tab.table.size() * sizeof(float)));
+    c->bc_bytes = tab.table.size() * sizeof(float);
     HIPCHK(hipMemcpy(c->d_cls, blk.data(), c->N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_bc, tab.table.data(), tab.table.size() * sizeof(float), hipMemcpyHostToDevice));
     return build_tile_list(c);

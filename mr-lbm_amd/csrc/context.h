@@ -33,6 +33,7 @@ struct mrlbm_ctx {
     void* d_mom = nullptr;
     uint8_t* d_cls = nullptr;
     float* d_bc = nullptr;
+    size_t bc_bytes = 0;  // capacity of d_bc
     float* d_ghost[2][6] = {};
     size_t face_elems[6] = {};
     int cur = 0;  // ghost set read by the next step

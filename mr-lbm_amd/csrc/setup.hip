@@ -243,7 +243,13 @@ int mrlbm_set_geometry(mrlbm_ctx* c, const uint8_t* solid, int nspheres, const d
 
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!c->d_cls) HIPCHK(hipMalloc((void**)&c->d_cls, c->N));
-    if (!c->d_bc) HIPCHK(hipMalloc((void**)&c->d_bc, 256 * BC_STRIDE * sizeof(float)));
+    if (c->bc_bytes < table.size() * sizeof(float)) {  // a table from mrlbm_set_node_types may be smaller
+        (void)hipFree(c->d_bc);
+        c->d_bc = nullptr;
+        c->bc_bytes = 0;
+        HIPCHK(hipMalloc((void**)&c->d_bc, table.size() * sizeof(float)));
+        c->bc_bytes = table.size() * sizeof(float);
+    }
     HIPCHK(hipMemcpyAsync(c->d_bc, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
 
     const Extents e = extents_of(c);

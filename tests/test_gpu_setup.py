@@ -80,3 +80,30 @@ def test_geometry_rejects_bad_spheres():
             lat.set_geometry(None, [(1.0, 2.0, float("nan"))], [2.0])
         with pytest.raises(mrlbm.MrlbmError):
             lat.set_geometry(None, [(1.0, 2.0, 3.0)], [-1.0])
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_geometry_after_node_types_and_back(q):
+    """A class table from mrlbm_set_node_types holds only that map's classes;
+    mrlbm_set_geometry afterwards needs the full octant table, and a node-type
+    map after that must replace the device classification again."""
+    shape = (32, 24, 16)
+    walls = (0, 1, 0)
+    centres, radii = [(12.0, 11.5, 8.0)], [4.5]
+    say("reuse", q)
+    with _lattice(shape, q, walls) as a, _lattice(shape, q, walls) as b, _lattice(shape, q, walls) as c:
+        a.set_node_types(geometry.classify(shape, walls))  # two wall classes
+        a.set_geometry(None, centres, radii)
+        b.set_geometry(None, centres, radii)
+        assert a.info()["n_bc_classes"] == b.info()["n_bc_classes"]
+        for lat in (a, b):
+            lat.init_uniform()
+            lat.step(30)
+        assert np.isfinite(a.moments()).all()
+        assert np.array_equal(a.moments(), b.moments())
+        a.set_node_types(geometry.classify(shape, walls))
+        c.set_node_types(geometry.classify(shape, walls))
+        for lat in (a, c):
+            lat.init_uniform()
+            lat.step(30)
+        assert np.array_equal(a.moments(), c.moments())
