@@ -23,6 +23,8 @@
 #include <stdint.h>
 #include <utility>
 
+#include "static_for.h"
+
 namespace mrlbm {
 
 // Global-address-space view of a device pointer. Pointers that kernels read
@@ -99,62 +101,125 @@ struct Node {
 // returned as g_i = f_i - w_i RHO_0 = w_i drho + w_i rho e_i (e_i = bracket - 1).
 // Opposite populations share the even part of the bracket, so each pair
 // costs two adds and two FMAs once the rho-scaled moments are formed.
+//
+// The work is split in two: recon_terms forms the rho-scaled moments once
+// per node, recon_pop<Q, I> evaluates one population from them. A kernel
+// that needs only a few populations of a node (the ones leaving a tile
+// face) evaluates just those from terms staged in LDS, with the same
+// operation tree, so the values are bit-identical to reconstruct()'s.
+struct Recon {
+    float ex, ey, ez;       // R (-(Mxx+Myy+Mzz)/3 + M_aa)
+    float ryy, rzz;         // R Myy, R Mzz
+    float rxy, rxz, ryz;    // R M_ab
+    float rux, ruy, ruz;    // R U_a
+    float d1, d2, d3;       // w_k drho for |c|^2 = 1, 2, 3
+    static constexpr int N = 14;
+    __device__ __forceinline__ float& at(int k) {
+        float* f[N] = {&ex, &ey, &ez, &ryy, &rzz, &rxy, &rxz, &ryz, &rux, &ruy, &ruz, &d1, &d2, &d3};
+        return *f[k];
+    }
+};
+
+// Terms of recon_pop<Q, I> that population I reads (bit k = Recon::at(k)).
+template <int Q>
+__host__ __device__ constexpr uint32_t recon_uses(int i) {
+    const int cx = cxv(i), cy = cyv(i), cz = czv(i);
+    const int n = (cx != 0) + (cy != 0) + (cz != 0);
+    uint32_t m = 0;
+    if (n == 1) {
+        const int a = cx ? 0 : (cy ? 1 : 2);
+        m = (1u << a) | (1u << (8 + a)) | (1u << 11);
+    } else if (n == 2) {
+        if (cz == 0) m = (1u << 0) | (1u << 3) | (1u << 5) | (1u << 8) | (1u << 9);
+        else if (cy == 0) m = (1u << 0) | (1u << 4) | (1u << 6) | (1u << 8) | (1u << 10);
+        else m = (1u << 1) | (1u << 4) | (1u << 7) | (1u << 9) | (1u << 10);
+        m |= 1u << 12;
+    } else if (n == 3) {
+        m = (1u << 0) | (1u << 3) | (1u << 4) | (1u << 5) | (1u << 6) | (1u << 7) | (1u << 8) | (1u << 9) |
+            (1u << 10) | (1u << 13);
+    }
+    return m;
+}
+
+template <int Q>
+__device__ __forceinline__ float recon_re0(const Node& n) {
+    return n.rho * (-(1.0f / 3.0f) * ((n.mxx + n.myy) + n.mzz));
+}
+
+template <int Q>
+__device__ __forceinline__ void recon_terms(const Node& n, float re0, Recon& t) {
+    constexpr float w1 = (float)VS<Q>::wd(1), w2 = (float)VS<Q>::wd(7);
+    const float R = n.rho;
+    t.rux = R * n.ux;
+    t.ruy = R * n.uy;
+    t.ruz = R * n.uz;
+    t.ex = re0 + R * n.mxx;
+    t.ey = re0 + (t.ryy = R * n.myy);
+    t.ez = re0 + (t.rzz = R * n.mzz);
+    t.rxy = R * n.mxy;
+    t.rxz = R * n.mxz;
+    t.ryz = R * n.myz;
+    t.d1 = w1 * n.drho;
+    t.d2 = w2 * n.drho;
+    if constexpr (Q == 27) t.d3 = (float)VS<Q>::wd(26) * n.drho;
+    else t.d3 = 0.0f;
+}
+
+// Population I (1 <= I < Q) from the terms.
+template <int Q, int I>
+__device__ __forceinline__ float recon_pop(const Recon& t) {
+    constexpr int cx = cxv(I), cy = cyv(I), cz = czv(I);
+    constexpr int nz = (cx != 0) + (cy != 0) + (cz != 0);
+    constexpr float w1 = (float)VS<Q>::wd(1), w2 = (float)VS<Q>::wd(7);
+    if constexpr (nz == 1) {
+        const float e = cx ? t.ex : (cy ? t.ey : t.ez);
+        const float u = cx ? t.rux : (cy ? t.ruy : t.ruz);
+        return __builtin_fmaf(w1, (cx + cy + cz) > 0 ? e + u : e - u, t.d1);
+    } else if constexpr (nz == 2) {
+        // plane ab (a < b): e_ab = (re0 + R M_aa) + R M_bb; populations
+        // (+a,+b), (-a,-b) use e_ab + R M_ab and R U_a + R U_b; (+a,-b),
+        // (-a,+b) use e_ab - R M_ab and R U_a - R U_b
+        constexpr int ca = cx ? cx : cy;
+        constexpr int cb = cz ? cz : cy;
+        float e, rab, ua, ub;
+        if constexpr (cz == 0) e = t.ex + t.ryy, rab = t.rxy, ua = t.rux, ub = t.ruy;
+        else if constexpr (cy == 0) e = t.ex + t.rzz, rab = t.rxz, ua = t.rux, ub = t.ruz;
+        else e = t.ey + t.rzz, rab = t.ryz, ua = t.ruy, ub = t.ruz;
+        if constexpr (ca == cb) {
+            const float p = e + rab, s = ua + ub;
+            return __builtin_fmaf(w2, ca > 0 ? p + s : p - s, t.d2);
+        } else {
+            const float m = e - rab, d = ua - ub;
+            return __builtin_fmaf(w2, ca > 0 ? m + d : m - d, t.d2);
+        }
+    } else {
+        static_assert(Q == 27 && nz == 3, "corner populations exist in D3Q27 only");
+        constexpr float w3 = (float)VS<Q>::wd(26);
+        // x sign fixes the +/- of the final combination; the xy pair sign
+        // picks a/b, the (xz, yz) pair signs pick e1..e4 and o1..o4
+        const float exyz = (t.ex + t.ryy) + t.rzz;
+        const bool sxy = cx == cy, sxz = cx == cz;
+        const float ab = sxy ? exyz + t.rxy : exyz - t.rxy;
+        const float st = sxy ? t.rux + t.ruy : t.rux - t.ruy;
+        float e, o;
+        if (sxy) {
+            e = sxz ? ab + (t.rxz + t.ryz) : ab - (t.rxz + t.ryz);
+        } else {
+            e = sxz ? ab + (t.rxz - t.ryz) : ab - (t.rxz - t.ryz);
+        }
+        o = sxz ? st + t.ruz : st - t.ruz;
+        return __builtin_fmaf(w3, cx > 0 ? e + o : e - o, t.d3);
+    }
+}
+
 template <int Q>
 __device__ __forceinline__ void reconstruct(const Node& n, float* g) {
-    constexpr float w0 = (float)VS<Q>::wd(0), w1 = (float)VS<Q>::wd(1), w2 = (float)VS<Q>::wd(7);
-    const float R = n.rho;
-    const float re0 = R * (-(1.0f / 3.0f) * ((n.mxx + n.myy) + n.mzz));
-    const float rux = R * n.ux, ruy = R * n.uy, ruz = R * n.uz;
-    const float rxx = R * n.mxx, ryy = R * n.myy, rzz = R * n.mzz;
-    const float rxy = R * n.mxy, rxz = R * n.mxz, ryz = R * n.myz;
-    const float d0 = w0 * n.drho, d1 = w1 * n.drho, d2 = w2 * n.drho;
-    g[0] = __builtin_fmaf(w0, re0, d0);
-    const float ex = re0 + rxx, ey = re0 + ryy, ez = re0 + rzz;
-    g[1] = __builtin_fmaf(w1, ex + rux, d1);
-    g[2] = __builtin_fmaf(w1, ex - rux, d1);
-    g[3] = __builtin_fmaf(w1, ey + ruy, d1);
-    g[4] = __builtin_fmaf(w1, ey - ruy, d1);
-    g[5] = __builtin_fmaf(w1, ez + ruz, d1);
-    g[6] = __builtin_fmaf(w1, ez - ruz, d1);
-    const float exy = ex + ryy, exz = ex + rzz, eyz = ey + rzz;
-    {
-        const float p = exy + rxy, m = exy - rxy, s = rux + ruy, d = rux - ruy;
-        g[7] = __builtin_fmaf(w2, p + s, d2);
-        g[8] = __builtin_fmaf(w2, p - s, d2);
-        g[13] = __builtin_fmaf(w2, m + d, d2);
-        g[14] = __builtin_fmaf(w2, m - d, d2);
-    }
-    {
-        const float p = exz + rxz, m = exz - rxz, s = rux + ruz, d = rux - ruz;
-        g[9] = __builtin_fmaf(w2, p + s, d2);
-        g[10] = __builtin_fmaf(w2, p - s, d2);
-        g[15] = __builtin_fmaf(w2, m + d, d2);
-        g[16] = __builtin_fmaf(w2, m - d, d2);
-    }
-    {
-        const float p = eyz + ryz, m = eyz - ryz, s = ruy + ruz, d = ruy - ruz;
-        g[11] = __builtin_fmaf(w2, p + s, d2);
-        g[12] = __builtin_fmaf(w2, p - s, d2);
-        g[17] = __builtin_fmaf(w2, m + d, d2);
-        g[18] = __builtin_fmaf(w2, m - d, d2);
-    }
-    if constexpr (Q == 27) {
-        constexpr float w3 = (float)VS<Q>::wd(26);
-        const float d3 = w3 * n.drho;
-        const float exyz = exy + rzz;
-        const float a = exyz + rxy, b = exyz - rxy;
-        const float e1 = a + (rxz + ryz), e2 = a - (rxz + ryz), e3 = b + (rxz - ryz), e4 = b - (rxz - ryz);
-        const float s = rux + ruy, t = rux - ruy;
-        const float o1 = s + ruz, o2 = s - ruz, o3 = t + ruz, o4 = t - ruz;
-        g[19] = __builtin_fmaf(w3, e1 + o1, d3);  // (+,+,+)
-        g[20] = __builtin_fmaf(w3, e1 - o1, d3);  // (-,-,-)
-        g[21] = __builtin_fmaf(w3, e2 + o2, d3);  // (+,+,-)
-        g[22] = __builtin_fmaf(w3, e2 - o2, d3);  // (-,-,+)
-        g[23] = __builtin_fmaf(w3, e3 + o3, d3);  // (+,-,+)
-        g[24] = __builtin_fmaf(w3, e3 - o3, d3);  // (-,+,-)
-        g[26] = __builtin_fmaf(w3, e4 + o4, d3);  // (+,-,-)
-        g[25] = __builtin_fmaf(w3, e4 - o4, d3);  // (-,+,+)
-    }
+    constexpr float w0 = (float)VS<Q>::wd(0);
+    const float re0 = recon_re0<Q>(n);
+    Recon t;
+    recon_terms<Q>(n, re0, t);
+    g[0] = __builtin_fmaf(w0, re0, w0 * n.drho);
+    static_for<Q - 1>([&](auto I) { g[decltype(I)::value + 1] = recon_pop<Q, decltype(I)::value + 1>(t); });
 }
 
 // Moments of the streamed populations at a bulk node

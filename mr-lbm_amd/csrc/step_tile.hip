@@ -71,6 +71,9 @@ __device__ __forceinline__ void halo_put(int lane, float v, float* s) {
     if (chunk_cell<Q, BX, BY, BZ, true, K>(lane, u, r)) s[c.lds0 + u * c.su + r * c.sv] = v;
 }
 
+// Outgoing chunk K: the lane's face node evaluates the chunk's population
+// from the reconstruction terms the node staged in LDS (term k of the node
+// at base nb is s[k * E + nb]) and writes it to the ghost plane.
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int K>
 __device__ __forceinline__ void halo_store(const StepArgs& a, float* const* gout, int lane, int tile, int bx, int by,
                                            int bz, const float* s, const uint8_t* s_cls) {
@@ -78,6 +81,7 @@ __device__ __forceinline__ void halo_store(const StepArgs& a, float* const* gout
     constexpr int QF = VS<Q>::QF;
     constexpr int Bu = face_bu<BX, BY, BZ>(c.face);
     constexpr int Bv = face_bv<BX, BY, BZ>(c.face);
+    using G = HaloGeom<BX, BY, BZ>;
     // wall predicate of src/includeFiles/interface.h:6-47: a tile face on a
     // wall-axis domain boundary is not written
     bool skip = false;
@@ -91,17 +95,24 @@ __device__ __forceinline__ void halo_store(const StepArgs& a, float* const* gout
     const auto dst = gptr(gout[c.face]) + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)tile) + Bu * c.w0;
     int u, r;
     if (chunk_cell<Q, BX, BY, BZ, false, K>(lane, u, r)) {
+        // node of the face cell
+        constexpr int ax = c.face / 2;
+        constexpr int fix = (c.face & 1) ? (ax == 0 ? BX : (ax == 1 ? BY : BZ)) - 1 : 0;
+        const int v = c.w0 + r;
+        const int nx = ax == 0 ? fix : u;
+        const int ny = ax == 1 ? fix : (ax == 0 ? u : v);
+        const int nz = ax == 2 ? fix : v;
         if constexpr (HAS_BC) {
-            // node of the face cell (solid nodes write no ghost values)
-            constexpr int ax = c.face / 2;
-            const int fix = (c.face & 1) ? (ax == 0 ? BX : (ax == 1 ? BY : BZ)) - 1 : 0;
-            const int v = c.w0 + r;
-            const int nx = ax == 0 ? fix : u;
-            const int ny = ax == 1 ? fix : (ax == 0 ? u : v);
-            const int nz = ax == 2 ? fix : v;
-            if (s_cls[nx + BX * (ny + BY * nz)] == 255u) return;
+            if (s_cls[nx + BX * (ny + BY * nz)] == 255u) return;  // solid nodes write no ghost values
         }
-        dst[(uint32_t)lane] = s[c.lds0 + u * c.su + r * c.sv];
+        const int nb = nx + G::DX * (ny + G::DY * nz);
+        constexpr uint32_t uses = recon_uses<Q>(c.pop);
+        Recon t;
+        static_for<Recon::N>([&](auto KK) {
+            constexpr int k = decltype(KK)::value;
+            t.at(k) = ((uses >> k) & 1u) ? s[k * G::E + nb] : 0.0f;
+        });
+        dst[(uint32_t)lane] = recon_pop<Q, c.pop>(t);
     }
 }
 
@@ -254,17 +265,21 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gou
     typename Codec<S>::T q[10];
     const uint32_t node_id = (uint32_t)(bx * BX + tx) + (uint32_t)a.nx * ((uint32_t)(by * BY + ty) + (uint32_t)a.ny * (uint32_t)(bz * BZ + tz));
     quantize<S>(m, a, step, node_id, q);
-    reconstruct<Q>(m, g);
+    Recon rt;
+    recon_terms<Q>(m, recon_re0<Q>(m), rt);
     const size_t mb = (size_t)tile * 10 * T + t;
     if (!solid) store_node<Q, BX, BY, BZ, S>(a, mb, q);
 
-    // populations leaving the tile: stage in LDS, then whole waves write
-    // the ghost planes (popSave.inc) in contiguous runs
+    // populations leaving the tile (popSave.inc): every node stages its
+    // reconstruction terms in LDS, then whole waves evaluate and write the
+    // ghost planes in contiguous runs, each lane only the one population
+    // of its chunk (the other populations of a face node stay unevaluated)
+    constexpr int NT = Q == 27 ? Recon::N : Recon::N - 1;  // d3 is D3Q27-only
+    static_assert(NT <= Q - 1, "terms fit the population arrays");
     __syncthreads();
-    static_for<Q - 1>([&](auto I) {
-        constexpr int i = decltype(I)::value + 1;
-        constexpr int wo = (cxv(i) > 0) + G::DX * ((cyv(i) > 0) + G::DY * (czv(i) > 0));
-        s[(i - 1) * E + base + wo] = g[i];
+    static_for<NT>([&](auto KK) {
+        constexpr int k = decltype(KK)::value;
+        s[k * E + base] = rt.at(k);
     });
     __syncthreads();
     dispatch<NW>(w, [&](auto WW) {
