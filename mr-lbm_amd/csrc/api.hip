@@ -83,6 +83,7 @@ int mrlbm_create(const mrlbm_params* pp, mrlbm_ctx** out) {
     const mrlbm_params& p = *pp;
     if (p.q != 19 && p.q != 27) return fail(MRLBM_EINVAL, "q must be 19 or 27 (got %d)", p.q);
     if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0) return fail(MRLBM_EINVAL, "lattice extents must be positive");
+    if (p.nx >= (1 << 24)) return fail(MRLBM_EINVAL, "nx must be below 2^24");  // 24-bit node-key multiply
     if (!(p.tau > 0.5)) return fail(MRLBM_EINVAL, "tau must exceed 0.5 (got %g)", p.tau);
     if (p.storage != MRLBM_STORE_FP32 && p.storage != MRLBM_STORE_FP16 && p.storage != MRLBM_STORE_BF16)
         return fail(MRLBM_EINVAL, "unknown storage precision %d", p.storage);
@@ -134,13 +135,10 @@ int mrlbm_create(const mrlbm_params* pp, mrlbm_ctx** out) {
     c->ntiles = c->ntx * c->nty * c->ntz;
     c->qf = p.q == 19 ? 5 : 9;
     c->N = (size_t)p.nx * p.ny * p.nz;
-    // storage scales: binary16 keeps the expected magnitudes (|drho|, |M| up
-    // to ~0.1, |U| up to ~0.3 for Mach < 0.2) near 2^8..2^9, far from both
-    // its overflow (65504) and its subnormal range (< 2^-14); bfloat16 has
-    // FP32's exponent range and needs none
+    // storage scales (powers of two), the same constants the step kernels use
+    static_assert(MRLBM_STORE_FP16 == 1 && MRLBM_STORE_BF16 == 2, "storage codes");
     for (int m = 0; m < 10; m++) {
-        float s = 1.0f;
-        if (p.storage == MRLBM_STORE_FP16) s = (m >= 1 && m <= 3) ? 1024.0f : 4096.0f;
+        const float s = store_scale(p.storage, m);
         c->mscale[m] = s;
         c->mrscale[m] = 1.0f / s;
     }

@@ -15,6 +15,32 @@ namespace mrlbm {
 // random bits (one draw per value, counter-based, see sr_bits below); adding
 // them below the kept mantissa bits and truncating gives an unbiased
 // rounding, so per-step changes smaller than half an ulp are not lost.
+// Storage scale of moment k (0 drho, 1-3 U, 4-9 M), a power of two:
+// binary16 keeps the expected magnitudes (|drho|, |M| up to ~0.1, |U| up to
+// ~0.3 for Mach < 0.2) near 2^8..2^9, far from both its overflow (65504) and
+// its subnormal range (< 2^-14); bfloat16 has FP32's exponent range and is
+// stored unscaled. The step kernels use these constants; the host uses the
+// same function for encoding and read-back.
+__host__ __device__ constexpr float store_scale(int storage, int k) {
+    return storage == 1 ? ((k >= 1 && k <= 3) ? 1024.0f : 4096.0f) : 1.0f;
+}
+
+// binary16 value (low or high half of `b`) times a power-of-two scale in one
+// mixed-precision FMA: v_fma_mix_f32 converts the half exactly and the
+// addend -0 makes the result x * s bit for bit, signed zeros included (the
+// same value as a conversion followed by a multiply, one instruction less).
+// The scale is uniform (an SGPR operand).
+template <bool HI>
+__device__ __forceinline__ float f16_scaled(uint32_t b, float s) {
+    float r;
+    const float nz = -0.0f;
+    if constexpr (HI)
+        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(b), "s"(s), "v"(nz));
+    else
+        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(b), "s"(s), "v"(nz));
+    return r;
+}
+
 template <int S> struct Codec;
 template <> struct Codec<0> {
     using T = float;
@@ -25,9 +51,7 @@ template <> struct Codec<0> {
 };
 template <> struct Codec<1> {  // IEEE binary16, 13 dropped bits rounded stochastically
     using T = uint16_t;
-    __device__ __forceinline__ static float dec(T b, float rscale) {
-        return (float)__builtin_bit_cast(_Float16, b) * rscale;
-    }
+    __device__ __forceinline__ static float dec(uint32_t b, float rscale) { return f16_scaled<false>(b, rscale); }
     __device__ __forceinline__ static T enc(float v, float scale, uint32_t rnd) {
         const uint32_t u = __float_as_uint(v * scale);
         const float t = __uint_as_float((u + (rnd & 0x1FFFu)) & ~0x1FFFu);
@@ -36,16 +60,24 @@ template <> struct Codec<1> {  // IEEE binary16, 13 dropped bits rounded stochas
     template <typename P>
     __device__ __forceinline__ static float ld(P p, size_t i, float rscale) { return dec(__builtin_nontemporal_load(&p[i]), rscale); }
 };
-template <> struct Codec<2> {  // bfloat16, 16 dropped bits rounded stochastically
+// bfloat16, 16 dropped bits rounded stochastically. FP32's exponent range
+// needs no storage scale: the library's bfloat16 scales are 1, and the
+// codec ignores them (no multiplies by 1 in the step kernels).
+template <> struct Codec<2> {
     using T = uint16_t;
-    __device__ __forceinline__ static float dec(T b, float rscale) { return __uint_as_float((uint32_t)b << 16) * rscale; }
-    __device__ __forceinline__ static T enc(float v, float scale, uint32_t rnd) {
-        const uint32_t u = __float_as_uint(v * scale);
+    __device__ __forceinline__ static float dec(uint32_t b, float) { return __uint_as_float(b << 16); }
+    __device__ __forceinline__ static T enc(float v, float, uint32_t rnd) {
+        const uint32_t u = __float_as_uint(v);
         return (T)((u + (rnd & 0xFFFFu)) >> 16);
     }
     template <typename P>
     __device__ __forceinline__ static float ld(P p, size_t i, float rscale) { return dec(__builtin_nontemporal_load(&p[i]), rscale); }
 };
+
+// Register type of a fetched stored moment: the 16-bit formats are held
+// zero-extended in 32-bit registers (the zero extension is part of the load).
+template <int S> struct Fetched { using T = uint32_t; };
+template <> struct Fetched<0> { using T = float; };
 
 // Counter-based random bits for stochastic rounding: a 32-bit mix of the
 // node's global index, the time step and the lattice seed. The same node,

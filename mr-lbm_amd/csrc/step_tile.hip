@@ -131,7 +131,7 @@ struct TileIn {
     static constexpr int NW = BX * BY * BZ / 64;
     static constexpr int NIN = (int)ChunkTables<Q, BX, BY, BZ>::in.size();
     static constexpr int RMAX = (NIN + NW - 1) / NW;
-    typename Codec<S>::T m[10];
+    typename Fetched<S>::T m[10];
     float h[RMAX];
     uint32_t cls;
 };
@@ -182,7 +182,8 @@ __device__ __forceinline__ void tile_fetch(const StepArgs& a, const float* const
     // moments are read and written exactly once per step: non-temporal
     // accesses keep them from displacing the ghost-plane lines that
     // neighbouring tiles read (measured +8 % on 256^3 FP32)
-    for (int k = 0; k < 10; k++) in.m[k] = __builtin_nontemporal_load(&mom[mb + (size_t)k * T]);
+    for (int k = 0; k < 10; k++)
+        in.m[k] = (typename Fetched<S>::T)__builtin_nontemporal_load(&mom[mb + (size_t)k * T]);
     // wave W owns incoming chunks W, W + NW, ...; every wave fills all
     // RMAX slots (zeros past its last chunk) so that no slot index depends
     // on the wave (a wave-dependent index would move the array to scratch)
@@ -263,7 +264,13 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gou
     }
     collide(m, a.R);
     typename Codec<S>::T q[10];
-    const uint32_t node_id = (uint32_t)(bx * BX + tx) + (uint32_t)a.nx * ((uint32_t)(by * BY + ty) + (uint32_t)a.ny * (uint32_t)(bz * BZ + tz));
+    // global node index x + nx (y + ny z), mod 2^32: tz is uniform in a wave
+    // (waves cover whole x rows, BX * BY is a multiple of 64), so only
+    // tx + nx ty is per-lane arithmetic (24-bit multiply: nx < 2^24)
+    static_assert((BX * BY) % 64 == 0, "a wave spans whole xy-plane rows");
+    const uint32_t nxu = (uint32_t)a.nx, nxy = nxu * (uint32_t)a.ny;
+    const uint32_t zw = (uint32_t)(bz * BZ) + (uint32_t)__builtin_amdgcn_readfirstlane(tz);
+    const uint32_t node_id = ((uint32_t)(bx * BX) + nxu * (uint32_t)(by * BY) + nxy * zw) + ((uint32_t)tx + __umul24(nxu, (uint32_t)ty));
     quantize<S>(m, a, step, node_id, q);
     Recon rt;
     recon_terms<Q>(m, recon_re0<Q>(m), rt);

@@ -42,21 +42,15 @@ __device__ __forceinline__ void load_node(const StepArgs& a, size_t mb, Node& n)
     n.mzz = C::ld(mom, mb + 9 * T, a.mrscale[9]);
 }
 
-// Node moments from the raw stored values (fetched earlier).
+// Node moments from the raw stored values (fetched earlier; 16-bit formats
+// arrive zero-extended to 32 bits).
 template <int S>
-__device__ __forceinline__ void decode_node(const StepArgs& a, const typename Codec<S>::T* m, Node& n) {
+__device__ __forceinline__ void decode_node(const StepArgs& a, const typename Fetched<S>::T* m, Node& n) {
     using C = Codec<S>;
-    n.drho = C::dec(m[0], a.mrscale[0]);
+    float* v[10] = {&n.drho, &n.ux, &n.uy, &n.uz, &n.mxx, &n.mxy, &n.mxz, &n.myy, &n.myz, &n.mzz};
+#pragma unroll
+    for (int k = 0; k < 10; k++) *v[k] = C::dec(m[k], store_scale(S, k) == 1.0f ? 1.0f : 1.0f / store_scale(S, k));
     n.rho = a.rho0 + n.drho;
-    n.ux = C::dec(m[1], a.mrscale[1]);
-    n.uy = C::dec(m[2], a.mrscale[2]);
-    n.uz = C::dec(m[3], a.mrscale[3]);
-    n.mxx = C::dec(m[4], a.mrscale[4]);
-    n.mxy = C::dec(m[5], a.mrscale[5]);
-    n.mxz = C::dec(m[6], a.mrscale[6]);
-    n.myy = C::dec(m[7], a.mrscale[7]);
-    n.myz = C::dec(m[8], a.mrscale[8]);
-    n.mzz = C::dec(m[9], a.mrscale[9]);
 }
 
 // Round the post-collision moments to the storage format and replace them
@@ -106,8 +100,14 @@ __device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t st
         float t[10];
 #pragma unroll
         for (int k = 0; k < 10; k++) {
-            const uint32_t r = (((w[k / 4] >> (8 * (k % 4))) & 0xFFu) << SH) | HALF;
-            t[k] = __uint_as_float((__float_as_uint(*v[k] * a.mscale[k]) + r) & KEEP);
+            const uint32_t r = (((w[k / 4] >> (8 * (k % 4))) & 0xFFu) << SH);
+            // binary16: the power-of-two scale 2^e enters as e added to the
+            // exponent field (uniform), which equals the bits of v * 2^e
+            // for every normal v; zero and FP32-subnormal v become values
+            // below 2^-100 that the conversion below flushes to the same
+            // signed zero as before. bfloat16 is stored unscaled.
+            const uint32_t e = (__builtin_bit_cast(uint32_t, store_scale(S, k)) - 0x3F800000u) + HALF;
+            t[k] = __uint_as_float((__float_as_uint(*v[k]) + r + e) & KEEP);
         }
         if constexpr (S == 1) {
 #pragma unroll
@@ -116,14 +116,14 @@ __device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t st
                 const uint32_t b = __builtin_bit_cast(uint32_t, p);
                 q[k] = (uint16_t)(b & 0xFFFFu);
                 q[k + 1] = (uint16_t)(b >> 16);
-                *v[k] = (float)p[0] * a.mrscale[k];
-                *v[k + 1] = (float)p[1] * a.mrscale[k + 1];
+                *v[k] = f16_scaled<false>(b, 1.0f / store_scale(S, k));
+                *v[k + 1] = f16_scaled<true>(b, 1.0f / store_scale(S, k + 1));
             }
         } else {
 #pragma unroll
             for (int k = 0; k < 10; k++) {
                 q[k] = (uint16_t)(__float_as_uint(t[k]) >> 16);
-                *v[k] = t[k] * a.mrscale[k];
+                *v[k] = t[k];
             }
         }
         n.rho = a.rho0 + n.drho;
