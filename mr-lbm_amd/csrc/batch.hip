@@ -334,10 +334,12 @@ int mrlbm_batch_autotune(int device, int q, int storage, int nshapes, const int*
             if (!rc) rc = mrlbm_batch_add(b, c);
             if (c) ctxs.push_back(c);
         }
-        double ms = 0;
+        double ms = 0, again = 0;
         const int per = steps % 10 == 0 ? 10 : 1;
         if (!rc) rc = mrlbm_batch_time_steps(b, steps, per, &ms);  // warm-up + capture
         if (!rc) rc = mrlbm_batch_time_steps(b, steps, per, &ms);
+        if (!rc) rc = mrlbm_batch_time_steps(b, steps, per, &again);  // best of two timed runs
+        if (!rc && again < ms) ms = again;
         mrlbm_batch_destroy(b);
         for (auto* c : ctxs) mrlbm_destroy(c);
         if (rc) return rc;

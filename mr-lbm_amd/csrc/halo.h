@@ -1,17 +1,21 @@
 // Halo layout of the tile kernel's LDS and the compile-time tables that
 // move ghost-plane data between HBM and LDS.
 //
-// Population i of the tile lives in an LDS array of (BX+1)(BY+1)(BZ+1)
-// cells. A node n writes its outgoing population at n + [c_i > 0] and reads
-// its incoming one at n + [c_i < 0] (per axis), so both are the thread's
-// base address plus a compile-time offset: no wrap arithmetic, no branch.
-// The extra plane on the upwind side of each array holds the populations
-// that enter the tile; they are copied from the neighbour tiles' ghost
-// planes by whole waves ("chunks"), each chunk a window of whole rows of a
-// ghost plane (at most 64 cells, contiguous in HBM). When a source node lies outside
-// the tile along several axes, the value comes from the face of the last of
-// those axes in X, Y, Z order, which is the branch that wins in the
-// reference's popLoad.inc (src/includeFiles/popLoad.inc:2-86).
+// Population i of the tile lives in an LDS array of extents B_a + |c_ia|
+// per axis (PopLds). A node n writes its outgoing population at
+// n + [c_i > 0] and reads its incoming one at n + [c_i < 0] (per axis), so
+// both are a base address (one per pair of x, y extents) plus a
+// compile-time offset: no wrap arithmetic, no branch. The extra plane on the
+// upwind side of an array holds the populations that enter the tile; they
+// are copied from the neighbour tiles' ghost planes by whole waves
+// ("chunks"), each chunk a window of whole rows of a ghost plane (at most 64
+// cells, contiguous in HBM). When a source node lies outside the tile along
+// several axes, the value comes from the face of the last of those axes in
+// X, Y, Z order, which is the branch that wins in the reference's
+// popLoad.inc (src/includeFiles/popLoad.inc:2-86).
+// Populations that move along x only stay in their wave when x rows lie in
+// 16-lane DPP rows (BX = 8 or 16): they are shifted between lanes, and their
+// LDS array is only the entering plane.
 #pragma once
 
 #include <array>
@@ -38,6 +42,9 @@ struct Chunk {
     int lds0;        // LDS index (floats) of cell (0, w0) under the map below
     int su, sv;      // LDS index = lds0 + u * su + (v - w0) * sv
 };
+// Incoming chunks index the population arrays (PopLds); outgoing chunks
+// index the face node in the staged-term layout, node base
+// x + (BX+1) (y + (BY+1) z) of term array k at k * HaloGeom::E.
 
 template <int BX, int BY, int BZ>
 struct HaloGeom {
@@ -45,6 +52,35 @@ struct HaloGeom {
     static constexpr int E = DX * DY * DZ;  // cells per population array
     static constexpr int B[3] = {BX, BY, BZ};
     static constexpr int STRIDE[3] = {1, DX, DX * DY};
+};
+
+// LDS arrays of the populations: extents B_a + |c_a| (the entering plane on
+// the upwind side), or the entering x plane only for populations that move
+// along x alone when BX = 8 or 16 (DPP lane shifts carry the rest).
+template <int Q, int BX, int BY, int BZ>
+struct PopLds {
+    static constexpr bool DPP_X = BX == 8 || BX == 16;
+    __host__ __device__ static constexpr bool x_only(int i) {
+        return DPP_X && cyv(i) == 0 && czv(i) == 0 && cxv(i) != 0;
+    }
+    __host__ __device__ static constexpr int dx(int i) { return x_only(i) ? 1 : BX + (cxv(i) != 0); }
+    __host__ __device__ static constexpr int dy(int i) { return BY + (cyv(i) != 0); }
+    __host__ __device__ static constexpr int dz(int i) { return BZ + (czv(i) != 0); }
+    __host__ __device__ static constexpr int size(int i) { return dx(i) * dy(i) * dz(i); }
+    __host__ __device__ static constexpr int off(int i) {
+        int o = 0;
+        for (int j = 1; j < i; j++) o += size(j);
+        return o;
+    }
+    __host__ __device__ static constexpr int total() { return off(Q); }
+    // stride of array axis a
+    __host__ __device__ static constexpr int stride(int i, int a) {
+        return a == 0 ? (x_only(i) ? 0 : 1) : (a == 1 ? dx(i) : dx(i) * dy(i));
+    }
+    // cell at array coordinates e (x coordinate ignored for x-only arrays)
+    __host__ __device__ static constexpr int cell(int i, int ex, int ey, int ez) {
+        return off(i) + (x_only(i) ? 0 : ex) + dx(i) * (ey + dy(i) * ez);
+    }
 };
 
 __host__ __device__ constexpr int face_slot(int face, int pop, int qf) {
@@ -142,12 +178,13 @@ constexpr int incoming_chunks_of(int i, Chunk* out) {
                 ch.uhi = hi[ua];
                 ch.vlo = vlo;
                 ch.vhi = vhi;
-                ch.su = mv[ua] ? G::STRIDE[ua] : 0;
-                ch.sv = mv[va] ? G::STRIDE[va] : 0;
+                using P = PopLds<Q, BX, BY, BZ>;
+                ch.su = mv[ua] ? P::stride(i, ua) : 0;
+                ch.sv = mv[va] ? P::stride(i, va) : 0;
                 // LDS cell of face cell (0, w0): moving axes at source 0 / w0
                 int e[3] = {e0[0], e0[1], e0[2]};
                 if (mv[va]) e[va] += w0;
-                ch.lds0 = (i - 1) * G::E + e[0] + G::DX * (e[1] + G::DY * e[2]);
+                ch.lds0 = P::cell(i, e[0], e[1], e[2]);
                 out[n] = ch;
             }
             n++;
@@ -172,7 +209,7 @@ constexpr std::array<Chunk, n_incoming_chunks<Q, BX, BY, BZ>()> incoming_chunks(
 }
 
 // Outgoing chunks: face f of this tile, slot s, all face nodes; the LDS
-// index is the node's write position of that population.
+// index is the face node's base in the staged-term layout.
 template <int Q, int BX, int BY, int BZ>
 constexpr int outgoing_chunks_impl(Chunk* out) {
     using G = HaloGeom<BX, BY, BZ>;
@@ -185,7 +222,6 @@ constexpr int outgoing_chunks_impl(Chunk* out) {
         const int rows = window_rows<BX, BY, BZ>(face);
         for (int s = 0; s < QF; s++) {
             const int i = face_pop(face, s);
-            const int c[3] = {cvec(i, 0), cvec(i, 1), cvec(i, 2)};
             for_windows(bv, rows, 0, bv - 1, [&](int w0, int vlo, int vhi) {
                 if (out) {
                     Chunk ch{};
@@ -198,12 +234,11 @@ constexpr int outgoing_chunks_impl(Chunk* out) {
                     ch.uhi = G::B[ua] - 1;
                     ch.vlo = vlo;
                     ch.vhi = vhi;
+                    // staged-term base of the face node (u, v) = (0, w0)
                     int node[3] = {0, 0, 0};
                     node[a] = (face & 1) ? G::B[a] - 1 : 0;
                     node[va] = w0;
-                    int e[3] = {0, 0, 0};
-                    for (int b = 0; b < 3; b++) e[b] = node[b] + (c[b] > 0 ? 1 : 0);
-                    ch.lds0 = (i - 1) * G::E + e[0] + G::DX * (e[1] + G::DY * e[2]);
+                    ch.lds0 = node[0] + G::DX * (node[1] + G::DY * node[2]);
                     ch.su = G::STRIDE[ua];
                     ch.sv = G::STRIDE[va];
                     out[n] = ch;

@@ -95,17 +95,17 @@ __device__ __forceinline__ void halo_store(const StepArgs& a, float* const* gout
     const auto dst = gptr(gout[c.face]) + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)tile) + Bu * c.w0;
     int u, r;
     if (chunk_cell<Q, BX, BY, BZ, false, K>(lane, u, r)) {
-        // node of the face cell
-        constexpr int ax = c.face / 2;
-        constexpr int fix = (c.face & 1) ? (ax == 0 ? BX : (ax == 1 ? BY : BZ)) - 1 : 0;
-        const int v = c.w0 + r;
-        const int nx = ax == 0 ? fix : u;
-        const int ny = ax == 1 ? fix : (ax == 0 ? u : v);
-        const int nz = ax == 2 ? fix : v;
         if constexpr (HAS_BC) {
-            if (s_cls[nx + BX * (ny + BY * nz)] == 255u) return;  // solid nodes write no ghost values
+            // node of the face cell; solid nodes write no ghost values
+            constexpr int ax = c.face / 2;
+            constexpr int fix = (c.face & 1) ? (ax == 0 ? BX : (ax == 1 ? BY : BZ)) - 1 : 0;
+            const int v = c.w0 + r;
+            const int nx = ax == 0 ? fix : u;
+            const int ny = ax == 1 ? fix : (ax == 0 ? u : v);
+            const int nz = ax == 2 ? fix : v;
+            if (s_cls[nx + BX * (ny + BY * nz)] == 255u) return;
         }
-        const int nb = nx + G::DX * (ny + G::DY * nz);
+        const int nb = c.lds0 + u * c.su + r * c.sv;  // staged-term base of the face node
         constexpr uint32_t uses = recon_uses<Q>(c.pop);
         Recon t;
         static_for<Recon::N>([&](auto KK) {
@@ -116,11 +116,20 @@ __device__ __forceinline__ void halo_store(const StepArgs& a, float* const* gout
     }
 }
 
-// LDS of one tile update: population arrays with halo, node classes.
-template <int Q, int BX, int BY, int BZ>
+// Terms staged per node for the outgoing ghost values (d3 is D3Q27-only).
+template <int Q>
+constexpr int staged_terms() { return Q == 27 ? Recon::N : Recon::N - 1; }
+
+// LDS of one tile update: the population arrays (PopLds), reused for the
+// staged terms (term k of node base b at k * E + b), and the node classes
+// (boundary body only: the bulk body needs none, which keeps D3Q19 8x8x8 at
+// 40,832 bytes, four workgroups per CU).
+template <int Q, int BX, int BY, int BZ, bool HAS_BC>
 struct TileLds {
-    float s[(Q - 1) * HaloGeom<BX, BY, BZ>::E];
-    uint8_t cls[BX * BY * BZ];
+    static constexpr int NP = PopLds<Q, BX, BY, BZ>::total();
+    static constexpr int NS = staged_terms<Q>() * HaloGeom<BX, BY, BZ>::E;
+    float s[NP > NS ? NP : NS];
+    uint8_t cls[HAS_BC ? BX * BY * BZ : 4];
 };
 
 // HBM reads of one tile update (class bytes, stored moments, this wave's
@@ -206,7 +215,7 @@ __device__ __forceinline__ void tile_fetch(const StepArgs& a, const float* const
 // (before the first barrier): it issues the next tile's fetch.
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S, typename Ahead>
 __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gout, uint32_t step, int tile,
-                                            TileLds<Q, BX, BY, BZ>& L, const TileIn<Q, BX, BY, BZ, S>& in,
+                                            TileLds<Q, BX, BY, BZ, HAS_BC>& L, const TileIn<Q, BX, BY, BZ, S>& in,
                                             Ahead&& ahead) {
     constexpr int T = BX * BY * BZ;
     constexpr int NW = T / 64;
@@ -240,19 +249,45 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gou
         });
     });
 
+    // population arrays: node base per (x extent, y extent) pair of the
+    // arrays, so every access is a base register plus an immediate
+    using P = PopLds<Q, BX, BY, BZ>;
+    auto pbase = [&](auto I) {
+        constexpr int i = decltype(I)::value + 1;
+        return tx + P::dx(i) * (ty + P::dy(i) * tz);
+    };
     float g[Q];
     reconstruct<Q>(n, g);
     static_for<Q - 1>([&](auto I) {
         constexpr int i = decltype(I)::value + 1;
-        constexpr int wo = (cxv(i) > 0) + G::DX * ((cyv(i) > 0) + G::DY * (czv(i) > 0));
-        s[(i - 1) * E + base + wo] = g[i];
+        if constexpr (!P::x_only(i)) {
+            constexpr int wo = P::cell(i, cxv(i) > 0, cyv(i) > 0, czv(i) > 0);
+            s[pbase(I) + wo] = g[i];
+        }
+    });
+    // populations moving along x only: from the x neighbour's lane (row_shr:1
+    // for +x, row_shl:1 for -x: x rows lie inside 16-lane DPP rows); the
+    // first node of a row takes the entering value from LDS after the barrier
+    float gx[Q];
+    static_for<Q - 1>([&](auto I) {
+        constexpr int i = decltype(I)::value + 1;
+        if constexpr (P::x_only(i)) {
+            constexpr int ctrl = cxv(i) > 0 ? 0x111 : 0x101;
+            gx[i] = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(g[i]), ctrl, 0xF, 0xF, false));
+        }
     });
     ahead();
     __syncthreads();
     static_for<Q - 1>([&](auto I) {
         constexpr int i = decltype(I)::value + 1;
-        constexpr int ro = (cxv(i) < 0) + G::DX * ((cyv(i) < 0) + G::DY * (czv(i) < 0));
-        g[i] = s[(i - 1) * E + base + ro];
+        if constexpr (P::x_only(i)) {
+            const bool edge = cxv(i) > 0 ? tx == 0 : tx == BX - 1;
+            const float h = s[P::off(i) + ty + BY * tz];
+            g[i] = edge ? h : gx[i];
+        } else {
+            constexpr int ro = P::cell(i, cxv(i) < 0, cyv(i) < 0, czv(i) < 0);
+            g[i] = s[pbase(I) + ro];
+        }
     });
 
     Node m;
@@ -281,8 +316,7 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gou
     // reconstruction terms in LDS, then whole waves evaluate and write the
     // ghost planes in contiguous runs, each lane only the one population
     // of its chunk (the other populations of a face node stay unevaluated)
-    constexpr int NT = Q == 27 ? Recon::N : Recon::N - 1;  // d3 is D3Q27-only
-    static_assert(NT <= Q - 1, "terms fit the population arrays");
+    constexpr int NT = staged_terms<Q>();
     __syncthreads();
     static_for<NT>([&](auto KK) {
         constexpr int k = decltype(KK)::value;
@@ -305,9 +339,25 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gou
 // tiles with this lookahead: workgroup turnover hides the load latency
 // better) and, for lattices with boundary nodes, one launch over the
 // all-bulk tiles and one over the rest.
+// Waves per SIMD the LDS of a workgroup allows (160 KB per CU, 4 SIMDs),
+// capped at 8. The bulk body asks the register allocator for that many
+// waves, so that a shape whose LDS admits four 512-thread workgroups per CU
+// is not held back by scalar registers (the step arguments alone take ~70
+// of them; a few are kept in VGPR lanes instead). The boundary body keeps
+// its registers (a tighter budget would spill VGPRs to memory).
+template <int Q, int BX, int BY, int BZ, bool HAS_BC>
+constexpr int lds_waves_per_simd() {
+    constexpr int bytes = (int)sizeof(TileLds<Q, BX, BY, BZ, HAS_BC>);
+    constexpr int granule = 512;
+    constexpr int groups = (160 * 1024) / (((bytes + granule - 1) / granule) * granule);
+    constexpr int w = groups * (BX * BY * BZ / 64) / 4;
+    return w > 8 ? 8 : (w < 1 ? 1 : w);
+}
+
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
-__global__ __launch_bounds__(BX* BY* BZ) void k_step_tile(StepArgs a, const uint32_t* __restrict__ list, int n) {
-    __shared__ TileLds<Q, BX, BY, BZ> L;
+__global__ __launch_bounds__(BX* BY* BZ) __attribute__((amdgpu_waves_per_eu(HAS_BC ? 1 : lds_waves_per_simd<Q, BX, BY, BZ, HAS_BC>())))
+void k_step_tile(StepArgs a, const uint32_t* __restrict__ list, int n) {
+    __shared__ TileLds<Q, BX, BY, BZ, HAS_BC> L;
     using In = TileIn<Q, BX, BY, BZ, S>;
     auto tile_at = [&](int i) { return list ? (int)__builtin_amdgcn_readfirstlane(list[i]) : i; };
     int i = blockIdx.x;
@@ -337,7 +387,7 @@ __global__ __launch_bounds__(BX* BY* BZ) void k_step_tile(StepArgs a, const uint
 template <int Q, int BX, int BY, int BZ, int S, bool HAS_BC>
 __global__ __launch_bounds__(BX* BY* BZ) void k_step_batch(const InstArgs* __restrict__ tab,
                                                            const uint32_t* __restrict__ work, int nwork, int k) {
-    __shared__ TileLds<Q, BX, BY, BZ> L;
+    __shared__ TileLds<Q, BX, BY, BZ, HAS_BC> L;
     using In = TileIn<Q, BX, BY, BZ, S>;
     const int per = (nwork + (int)gridDim.x - 1) / (int)gridDim.x;
     const int begin = (int)blockIdx.x * per;

@@ -1,6 +1,10 @@
 // Host check of the tile kernel's ghost-chunk tables (mr-lbm_amd/csrc/halo.h):
 // simulates every lane of every chunk and compares with a brute-force map.
 //
+// LDS geometry, restated here independently of halo.h: population i has an
+// array of extents (B_a + |c_ia|) per axis, stored one after another in
+// population order, x fastest; when BX is 8 or 16, a population moving
+// along x only keeps just its entering x plane (extent 1 along x).
 // Incoming: for each node n and population i whose source n - c_i lies
 // outside the tile along a set A of axes, the LDS read cell n + [c_i < 0]
 // must receive, exactly once, the ghost value of face 2*win + [c_win > 0]
@@ -8,8 +12,9 @@
 // the owner tile at offset -c along A, slot of i, at the source's face
 // coordinates; no other cell may be written. Every load index must lie in
 // the owner's plane of that slot.
-// Outgoing: every face cell of every slot is written exactly once, from the
-// LDS cell where the face node wrote population i (node + [c_i > 0]).
+// Outgoing: every face cell of every slot is written exactly once, by a lane
+// whose LDS index is the face node's base x + (BX+1)(y + (BY+1) z) in the
+// staged-term layout.
 // Prints "ok" and exits 0 on success.
 #include <cstdio>
 #include <map>
@@ -31,6 +36,22 @@ static int failures = 0;
             failures++;                      \
         }                                    \
     } while (0)
+
+// LDS cell of population i at array coordinates e (the geometry above)
+template <int Q, int BX, int BY, int BZ>
+int pop_cell(int i, const int e[3]) {
+    const int B[3] = {BX, BY, BZ};
+    auto ext = [&](int j, int a) {
+        const int c[3] = {cxv(j), cyv(j), czv(j)};
+        const bool xonly = (BX == 8 || BX == 16) && c[0] != 0 && c[1] == 0 && c[2] == 0;
+        if (a == 0 && xonly) return 1;
+        return B[a] + (c[a] != 0 ? 1 : 0);
+    };
+    int off = 0;
+    for (int j = 1; j < i; j++) off += ext(j, 0) * ext(j, 1) * ext(j, 2);
+    const int ex = ext(i, 0) == 1 ? 0 : e[0];
+    return off + ex + ext(i, 0) * (e[1] + ext(i, 1) * e[2]);
+}
 
 template <int Q, int BX, int BY, int BZ>
 void check_shape() {
@@ -65,7 +86,9 @@ void check_shape() {
                     }
                     int e[3];
                     for (int a = 0; a < 3; a++) e[a] = n[a] + (c[a] < 0 ? 1 : 0);
-                    const int cell = (i - 1) * G::E + e[0] + G::DX * (e[1] + G::DY * e[2]);
+                    const int cell = pop_cell<Q, BX, BY, BZ>(i, e);
+                    CHECK(expect.find(cell) == expect.end(), "Q%d %dx%dx%d two sources for LDS cell %d", Q, BX, BY, BZ,
+                          cell);
                     expect[cell] = Src(face, d[0], d[1], d[2], face_slot(face, i, QF), own[ua], own[va]);
                 }
     }
@@ -113,10 +136,9 @@ void check_shape() {
             node[a] = (ch.face & 1) ? B[a] - 1 : 0;
             node[ua] = u;
             node[va] = v;
-            int e[3];
-            for (int b = 0; b < 3; b++) e[b] = node[b] + (c[b] > 0 ? 1 : 0);
-            const int cell = (i - 1) * G::E + e[0] + G::DX * (e[1] + G::DY * e[2]);
-            CHECK(ch.lds0 + u * ch.su + r * ch.sv == cell, "outgoing LDS cell mismatch Q%d pop %d", Q, i);
+            const int nodebase = node[0] + (BX + 1) * (node[1] + (BY + 1) * node[2]);
+            CHECK(ch.lds0 + u * ch.su + r * ch.sv == nodebase, "outgoing LDS cell mismatch Q%d pop %d", Q, i);
+            (void)c;
             out_written[std::make_tuple(ch.face, ch.slot, plane)]++;
         }
     }
