@@ -98,7 +98,7 @@ def batching_cavities(n_lat=64, steps=2000, seed=2024, check=4):
         spec.append((n, 0.5 + 3.0 * lid * n / re))
     dims = [(n, n, n) for n, _ in spec]
     shapes = ((8, 8, 8), (16, 8, 4), (32, 4, 4), (64, 4, 2))
-    tile, tune_ms = mrlbm.autotune(dims, shapes=shapes, steps=20)
+    tile, tune_ms = mrlbm.autotune(dims, shapes=shapes, steps=50)
     types = {n: geometry.cavity(n) for n in set(n for n, _ in spec)}
 
     def make(n, tau):
@@ -140,7 +140,7 @@ def batching_cavities(n_lat=64, steps=2000, seed=2024, check=4):
                     f"lid 0.05, {steps} steps each (+{gsteps} warm-up), seed {seed}",
         "total_nodes": nodes,
         "tile": list(tile),
-        "autotune_ms_20_steps": {"x".join(map(str, k)): v for k, v in tune_ms.items()},
+        "autotune_ms_50_steps": {"x".join(map(str, k)): v for k, v in tune_ms.items()},
         "batched_mlups": round(nodes * steps / (ms_b * 1e-3) / 1e6, 1),
         "sequential_mlups": round(nodes * steps / (ms_seq * 1e-3) / 1e6, 1),
         "speedup": round(ms_seq / ms_b, 3),
