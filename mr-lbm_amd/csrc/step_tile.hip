@@ -340,11 +340,10 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gou
 // better) and, for lattices with boundary nodes, one launch over the
 // all-bulk tiles and one over the rest.
 // Waves per SIMD the LDS of a workgroup allows (160 KB per CU, 4 SIMDs),
-// capped at 8. The bulk body asks the register allocator for that many
-// waves, so that a shape whose LDS admits four 512-thread workgroups per CU
-// is not held back by scalar registers (the step arguments alone take ~70
-// of them; a few are kept in VGPR lanes instead). The boundary body keeps
-// its registers (a tighter budget would spill VGPRs to memory).
+// capped at 8. The register allocator is asked for that many waves, so that
+// a shape whose LDS admits four 512-thread workgroups per CU is not held
+// back by scalar registers (the step arguments alone take ~70 of them; a
+// few are kept in VGPR lanes instead).
 template <int Q, int BX, int BY, int BZ, bool HAS_BC>
 constexpr int lds_waves_per_simd() {
     constexpr int bytes = (int)sizeof(TileLds<Q, BX, BY, BZ, HAS_BC>);
@@ -354,8 +353,17 @@ constexpr int lds_waves_per_simd() {
     return w > 8 ? 8 : (w < 1 ? 1 : w);
 }
 
+// Register budget of a step body: the bulk body, and the FP32 boundary body
+// with x rows of at most 16 nodes (it fits the 80 VGPRs of 6 waves per SIMD
+// without spilling), take the waves the LDS allows; other boundary bodies
+// keep the compiler's choice (a tighter budget spills VGPRs to memory).
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
-__global__ __launch_bounds__(BX* BY* BZ) __attribute__((amdgpu_waves_per_eu(HAS_BC ? 1 : lds_waves_per_simd<Q, BX, BY, BZ, HAS_BC>())))
+constexpr int step_body_waves() {
+    return (!HAS_BC || (S == 0 && BX <= 16)) ? lds_waves_per_simd<Q, BX, BY, BZ, HAS_BC>() : 1;
+}
+
+template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
+__global__ __launch_bounds__(BX* BY* BZ) __attribute__((amdgpu_waves_per_eu(step_body_waves<Q, BX, BY, BZ, HAS_BC, S>())))
 void k_step_tile(StepArgs a, const uint32_t* __restrict__ list, int n) {
     __shared__ TileLds<Q, BX, BY, BZ, HAS_BC> L;
     using In = TileIn<Q, BX, BY, BZ, S>;
@@ -385,7 +393,8 @@ void k_step_tile(StepArgs a, const uint32_t* __restrict__ list, int n) {
 // workgroups per CU); HAS_BC = true runs the tiles that hold boundary or
 // solid nodes, and tiles of lattices without a class map read class 0.
 template <int Q, int BX, int BY, int BZ, int S, bool HAS_BC>
-__global__ __launch_bounds__(BX* BY* BZ) void k_step_batch(const InstArgs* __restrict__ tab,
+__global__ __launch_bounds__(BX* BY* BZ) __attribute__((amdgpu_waves_per_eu(step_body_waves<Q, BX, BY, BZ, HAS_BC, S>())))
+void k_step_batch(const InstArgs* __restrict__ tab,
                                                            const uint32_t* __restrict__ work, int nwork, int k) {
     __shared__ TileLds<Q, BX, BY, BZ, HAS_BC> L;
     using In = TileIn<Q, BX, BY, BZ, S>;
