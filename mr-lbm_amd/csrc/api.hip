@@ -425,13 +425,13 @@ static int do_steps(mrlbm_ctx* c, int nsteps) {
         } else if (split) {
             // all-bulk tiles with the light body, the rest with boundaries;
             // both read the same ghost set and write disjoint data
-            e = launch_step_tile(c->p.q, c->bx, c->by, c->bz, false, c->p.storage, a, c->d_tiles, c->n_bulk_tiles, 0,
+            e = launch_step_tile(c->p.q, c->bx, c->by, c->bz, false, c->p.storage, a, c->d_tiles, c->n_bulk_tiles,
                                  c->stream);
             if (e == hipSuccess)
                 e = launch_step_tile(c->p.q, c->bx, c->by, c->bz, true, c->p.storage, a, c->d_tiles + c->n_bulk_tiles,
-                                     c->ntiles - c->n_bulk_tiles, 0, c->stream);
+                                     c->ntiles - c->n_bulk_tiles, c->stream);
         } else {
-            e = launch_step_tile(c->p.q, c->bx, c->by, c->bz, c->has_bc, c->p.storage, a, nullptr, c->ntiles, 0,
+            e = launch_step_tile(c->p.q, c->bx, c->by, c->bz, c->has_bc, c->p.storage, a, nullptr, c->ntiles,
                                  c->stream);
         }
         if (e != hipSuccess) return fail(MRLBM_EHIP, "step launch failed: %s", hipGetErrorString(e));

@@ -152,12 +152,11 @@ hipError_t launch_step_batch(int q, int bx, int by, int bz, int storage, bool ha
 hipError_t launch_batch_advance(InstArgs* tab, int ninst, int nsteps, hipStream_t s);
 
 // Launchers (step_tile.hip). Return hipError_t of the launch.
-// Step of the n tiles list[0..n) (tiles 0..n-1 when list is null) by
-// min(grid, n) workgroups walking entries blockIdx.x, + gridDim.x, ...;
-// grid <= 0: one tile per workgroup. has_bc = false only for tiles whose
-// nodes are all bulk.
+// Step of the n tiles list[0..n) (tiles 0..n-1 when list is null), one
+// workgroup per tile. has_bc = false only for tiles whose nodes are all
+// bulk.
 hipError_t launch_step_tile(int q, int bx, int by, int bz, bool has_bc, int storage, const StepArgs& a,
-                            const uint32_t* list, int n, int grid, hipStream_t s);
+                            const uint32_t* list, int n, hipStream_t s);
 bool tile_shape_supported(int q, int bx, int by, int bz, int storage);
 
 // Column kernel (step_column.hip): one (x,y) column per thread over a

@@ -332,13 +332,6 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gou
     });
 }
 
-// Tile step over entries blockIdx.x, + gridDim.x, ... of a tile list (the
-// tile indices themselves when `list` is null), n entries; each tile's
-// reads are issued while the previous tile computes. The library launches
-// one tile per block (measured faster on 256^3 than a resident grid walking
-// tiles with this lookahead: workgroup turnover hides the load latency
-// better) and, for lattices with boundary nodes, one launch over the
-// all-bulk tiles and one over the rest.
 // Waves per SIMD the LDS of a workgroup allows (160 KB per CU, 4 SIMDs),
 // capped at 8. The register allocator is asked for that many waves, so that
 // a shape whose LDS admits four 512-thread workgroups per CU is not held
@@ -362,29 +355,20 @@ constexpr int step_body_waves() {
     return (!HAS_BC || (S == 0 && BX <= 16)) ? lds_waves_per_simd<Q, BX, BY, BZ, HAS_BC>() : 1;
 }
 
+// Tile step: block b updates tile list[b] (tile b when `list` is null), one
+// tile per block: workgroup turnover hides the load latency better than a
+// resident grid walking tiles with a prefetch of the next one (measured on
+// 256^3), and the single-tile body holds fewer scalar registers. For
+// lattices with boundary nodes the library launches the all-bulk tiles and
+// the rest separately.
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
 __global__ __launch_bounds__(BX* BY* BZ) __attribute__((amdgpu_waves_per_eu(step_body_waves<Q, BX, BY, BZ, HAS_BC, S>())))
-void k_step_tile(StepArgs a, const uint32_t* __restrict__ list, int n) {
+void k_step_tile(StepArgs a, const uint32_t* __restrict__ list) {
     __shared__ TileLds<Q, BX, BY, BZ, HAS_BC> L;
-    using In = TileIn<Q, BX, BY, BZ, S>;
-    auto tile_at = [&](int i) { return list ? (int)__builtin_amdgcn_readfirstlane(list[i]) : i; };
-    int i = blockIdx.x;
-    int tile = tile_at(i);
-    In in;
+    const int tile = list ? (int)__builtin_amdgcn_readfirstlane(list[blockIdx.x]) : (int)blockIdx.x;
+    TileIn<Q, BX, BY, BZ, S> in;
     tile_fetch<Q, BX, BY, BZ, HAS_BC, S>(a, a.gin, tile, in);
-    while (true) {
-        const int ni = i + (int)gridDim.x;
-        const int next = ni < n ? tile_at(ni) : 0;
-        In nx;
-        tile_update<Q, BX, BY, BZ, HAS_BC, S>(a, a.gout, a.step, tile, L, in, [&]() {
-            if (ni < n) tile_fetch<Q, BX, BY, BZ, HAS_BC, S>(a, a.gin, next, nx);
-        });
-        if (ni >= n) break;
-        __syncthreads();  // LDS is reused by the next tile
-        in = nx;
-        i = ni;
-        tile = next;
-    }
+    tile_update<Q, BX, BY, BZ, HAS_BC, S>(a, a.gout, a.step, tile, L, in, []() {});
 }
 
 // Batched step: each block takes a contiguous range of work items (the
@@ -487,35 +471,33 @@ bool tile_shape_supported(int q, int bx, int by, int bz, int storage) {
 }
 
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
-static hipError_t launch_qsb(const StepArgs& a, const uint32_t* list, int n, int grid, hipStream_t s) {
+static hipError_t launch_qsb(const StepArgs& a, const uint32_t* list, int n, hipStream_t s) {
     constexpr int T = BX * BY * BZ;
-    if (grid <= 0 || grid > n) grid = n;
-    hipLaunchKernelGGL((k_step_tile<Q, BX, BY, BZ, HAS_BC, S>), dim3(grid), dim3(T), 0, s, a, list, n);
+    hipLaunchKernelGGL((k_step_tile<Q, BX, BY, BZ, HAS_BC, S>), dim3(n), dim3(T), 0, s, a, list);
     return hipGetLastError();
 }
 
 template <int Q, int BX, int BY, int BZ, int S>
-static hipError_t launch_qs(bool has_bc, const StepArgs& a, const uint32_t* list, int n, int grid, hipStream_t s) {
-    return has_bc ? launch_qsb<Q, BX, BY, BZ, true, S>(a, list, n, grid, s)
-                  : launch_qsb<Q, BX, BY, BZ, false, S>(a, list, n, grid, s);
+static hipError_t launch_qs(bool has_bc, const StepArgs& a, const uint32_t* list, int n, hipStream_t s) {
+    return has_bc ? launch_qsb<Q, BX, BY, BZ, true, S>(a, list, n, s) : launch_qsb<Q, BX, BY, BZ, false, S>(a, list, n, s);
 }
 
 template <int Q, int BX, int BY, int BZ>
-static hipError_t launch_q_shape(bool has_bc, int storage, const StepArgs& a, const uint32_t* list, int n, int grid,
+static hipError_t launch_q_shape(bool has_bc, int storage, const StepArgs& a, const uint32_t* list, int n,
                                  hipStream_t s) {
-    if (storage == 0) return launch_qs<Q, BX, BY, BZ, 0>(has_bc, a, list, n, grid, s);
-    if (storage == 1) return launch_qs<Q, BX, BY, BZ, 1>(has_bc, a, list, n, grid, s);
-    if (storage == 2) return launch_qs<Q, BX, BY, BZ, 2>(has_bc, a, list, n, grid, s);
+    if (storage == 0) return launch_qs<Q, BX, BY, BZ, 0>(has_bc, a, list, n, s);
+    if (storage == 1) return launch_qs<Q, BX, BY, BZ, 1>(has_bc, a, list, n, s);
+    if (storage == 2) return launch_qs<Q, BX, BY, BZ, 2>(has_bc, a, list, n, s);
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_step_tile(int q, int bx, int by, int bz, bool has_bc, int storage, const StepArgs& a,
-                            const uint32_t* list, int n, int grid, hipStream_t s) {
+                            const uint32_t* list, int n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
 #define X(A, B, C)                                                                              \
     if (bx == A && by == B && bz == C) {                                                        \
-        if (q == 19) return launch_q_shape<19, A, B, C>(has_bc, storage, a, list, n, grid, s); \
-        if (q == 27) return launch_q_shape<27, A, B, C>(has_bc, storage, a, list, n, grid, s); \
+        if (q == 19) return launch_q_shape<19, A, B, C>(has_bc, storage, a, list, n, s); \
+        if (q == 27) return launch_q_shape<27, A, B, C>(has_bc, storage, a, list, n, s); \
     }
     MRLBM_TILE_SHAPES(X)
 #undef X
