@@ -144,9 +144,11 @@ struct InstArgs {
 };
 
 // Batched step (step_tile.hip): one launch advances every lattice of the
-// work table by one step. work[i] = instance << 20 | tile. grid <= nwork
-// blocks; block b takes the contiguous items [b*per, (b+1)*per), per =
-// ceil(nwork/grid). has_bc = false only for tiles whose nodes are all bulk.
+// work table by one step. work[i] = instance << 20 | tile. grid <= 0 or
+// grid >= nwork: one item per block; otherwise grid blocks, block b taking
+// the contiguous items [b*per, (b+1)*per), per = ceil(nwork/grid), with the
+// next item's reads issued while the current one computes. has_bc = false
+// only for tiles whose nodes are all bulk.
 hipError_t launch_step_batch(int q, int bx, int by, int bz, int storage, bool has_bc, const InstArgs* tab,
                              const uint32_t* work, int nwork, int grid, int k, hipStream_t s);
 hipError_t launch_batch_advance(InstArgs* tab, int ninst, int nsteps, hipStream_t s);

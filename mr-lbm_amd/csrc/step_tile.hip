@@ -417,6 +417,23 @@ void k_step_batch(const InstArgs* __restrict__ tab,
     }
 }
 
+// Batched step with one work item per block (the default grid): like
+// k_step_tile, the single-item body holds far fewer registers than the
+// ranged kernel's loop with its prefetch (D3Q19 8x8x8 boundary body: ~76
+// instead of ~110 VGPRs, three workgroups per CU instead of two).
+template <int Q, int BX, int BY, int BZ, int S, bool HAS_BC>
+__global__ __launch_bounds__(BX* BY* BZ) __attribute__((amdgpu_waves_per_eu(step_body_waves<Q, BX, BY, BZ, HAS_BC, S>())))
+void k_step_batch1(const InstArgs* __restrict__ tab, const uint32_t* __restrict__ work, int k) {
+    __shared__ TileLds<Q, BX, BY, BZ, HAS_BC> L;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(work[blockIdx.x]);
+    const int inst = (int)(wv >> 20), tile = (int)(wv & 0xFFFFFu);
+    const InstArgs& ia = tab[inst];  // read in place (scalar loads)
+    const int p = (ia.cur0 + k) & 1;
+    TileIn<Q, BX, BY, BZ, S> in;
+    tile_fetch<Q, BX, BY, BZ, HAS_BC, S>(ia.a, ia.g[p], tile, in);
+    tile_update<Q, BX, BY, BZ, HAS_BC, S>(ia.a, ia.g[1 - p], ia.step0 + (uint32_t)k, tile, L, in, []() {});
+}
+
 __global__ void k_batch_advance(InstArgs* tab, int ninst, int nsteps) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ninst) {
@@ -531,7 +548,16 @@ hipError_t launch_init_ghosts(int q, int bx, int by, int bz, int storage, const 
 template <int Q, int BX, int BY, int BZ, bool HAS_BC>
 static hipError_t launch_batch_qb(int storage, const InstArgs* tab, const uint32_t* work, int nwork, int grid, int k,
                                   hipStream_t s) {
-    dim3 g(grid), b(BX * BY * BZ);
+    dim3 b(BX * BY * BZ);
+    if (grid <= 0 || grid >= nwork) {  // one item per block
+        dim3 g(nwork);
+        if (storage == 0) hipLaunchKernelGGL((k_step_batch1<Q, BX, BY, BZ, 0, HAS_BC>), g, b, 0, s, tab, work, k);
+        else if (storage == 1) hipLaunchKernelGGL((k_step_batch1<Q, BX, BY, BZ, 1, HAS_BC>), g, b, 0, s, tab, work, k);
+        else if (storage == 2) hipLaunchKernelGGL((k_step_batch1<Q, BX, BY, BZ, 2, HAS_BC>), g, b, 0, s, tab, work, k);
+        else return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
+    dim3 g(grid);
     if (storage == 0) hipLaunchKernelGGL((k_step_batch<Q, BX, BY, BZ, 0, HAS_BC>), g, b, 0, s, tab, work, nwork, k);
     else if (storage == 1) hipLaunchKernelGGL((k_step_batch<Q, BX, BY, BZ, 1, HAS_BC>), g, b, 0, s, tab, work, nwork, k);
     else if (storage == 2) hipLaunchKernelGGL((k_step_batch<Q, BX, BY, BZ, 2, HAS_BC>), g, b, 0, s, tab, work, nwork, k);
