@@ -98,7 +98,7 @@ def batching_cavities(n_lat=64, steps=2000, seed=2024, check=4):
         spec.append((n, 0.5 + 3.0 * lid * n / re))
     dims = [(n, n, n) for n, _ in spec]
     shapes = ((8, 8, 8), (16, 8, 4), (32, 4, 4), (64, 4, 2))
-    tile, tune_ms = mrlbm.autotune(dims, shapes=shapes, steps=50)
+    tile, tune_ms = mrlbm.autotune(dims, shapes=shapes, steps=50, walls=[(1, 1, 1)] * len(dims))
     types = {n: geometry.cavity(n) for n in set(n for n, _ in spec)}
 
     def make(n, tau):

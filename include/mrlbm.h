@@ -225,6 +225,11 @@ int mrlbm_batch_destroy(mrlbm_batch* b);
  * ms_out[s] = its time (-1 if the shape does not fit). */
 int mrlbm_batch_autotune(int device, int q, int storage, int nshapes, const int* shapes, int ninst, const int* dims,
                          int steps, int* best, double* ms_out);
+/* Same, with the lattices' wall axes (walls = 3*ninst ints, nonzero = wall;
+ * null = all periodic): wall lattices get device-classified domain walls,
+ * so the boundary tiles are timed with the boundary body. */
+int mrlbm_batch_autotune_walls(int device, int q, int storage, int nshapes, const int* shapes, int ninst,
+                               const int* dims, const int* walls, int steps, int* best, double* ms_out);
 
 /* ---------------------------------------------------------------------
  * Geometry (host): node-type maps in the reference encoding.

@@ -299,6 +299,11 @@ int mrlbm_batch_destroy(mrlbm_batch* b) {
 // divides every extent; report the fastest.
 int mrlbm_batch_autotune(int device, int q, int storage, int nshapes, const int* shapes, int ninst, const int* dims,
                          int steps, int* best, double* ms_out) {
+    return mrlbm_batch_autotune_walls(device, q, storage, nshapes, shapes, ninst, dims, nullptr, steps, best, ms_out);
+}
+
+int mrlbm_batch_autotune_walls(int device, int q, int storage, int nshapes, const int* shapes, int ninst,
+                               const int* dims, const int* walls, int steps, int* best, double* ms_out) {
     if (!shapes || !dims || !best || nshapes <= 0 || ninst <= 0 || steps <= 0)
         return fail(MRLBM_EINVAL, "bad argument");
     *best = -1;
@@ -328,8 +333,16 @@ int mrlbm_batch_autotune(int device, int q, int storage, int nshapes, const int*
             p.tau = 0.6;
             p.rho0 = 1.0;
             p.device = device;
+            if (walls) {
+                p.wall_x = walls[3 * i] != 0;
+                p.wall_y = walls[3 * i + 1] != 0;
+                p.wall_z = walls[3 * i + 2] != 0;
+            }
             mrlbm_ctx* c = nullptr;
             rc = mrlbm_create(&p, &c);
+            // wall axes: domain walls classified on the device, so the boundary
+            // tiles run the boundary body as in the real lattices
+            if (!rc && walls && (p.wall_x || p.wall_y || p.wall_z)) rc = mrlbm_set_geometry(c, nullptr, 0, nullptr);
             if (!rc) rc = mrlbm_init_uniform(c, 1.0, 0.0, 0.0, 0.0, MRLBM_INIT_EQUILIBRIUM);
             if (!rc) rc = mrlbm_batch_add(b, c);
             if (c) ctxs.push_back(c);

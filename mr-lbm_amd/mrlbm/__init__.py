@@ -286,13 +286,21 @@ class Batch:
 
 
 def autotune(dims, shapes=((8, 8, 8), (16, 8, 4), (32, 4, 4), (64, 4, 2)), q=19, storage=STORE_FP32, steps=20,
-             device=0):
-    """Fastest tile shape for batching lattices of extents `dims`; returns
-    (shape, {shape: ms or None})."""
+             device=0, walls=None):
+    """Fastest tile shape for batching lattices of extents `dims` (and wall
+    axes `walls`, one (wx, wy, wz) per lattice, default all periodic);
+    returns (shape, {shape: ms or None})."""
     sh = (ctypes.c_int * (3 * len(shapes)))(*[v for s in shapes for v in s])
     dm = (ctypes.c_int * (3 * len(dims)))(*[v for d in dims for v in d])
     best = ctypes.c_int()
     ms = np.zeros(len(shapes))
-    _lib.check(_lib.load().mrlbm_batch_autotune(device, q, storage, len(shapes), sh, len(dims), dm, steps,
-                                                ctypes.byref(best), ms.ctypes.data_as(_lib._dp)))
+    if walls is None:
+        _lib.check(_lib.load().mrlbm_batch_autotune(device, q, storage, len(shapes), sh, len(dims), dm, steps,
+                                                    ctypes.byref(best), ms.ctypes.data_as(_lib._dp)))
+    else:
+        if len(walls) != len(dims):
+            raise ValueError("one wall triple per lattice")
+        wl = (ctypes.c_int * (3 * len(walls)))(*[int(bool(v)) for w in walls for v in w])
+        _lib.check(_lib.load().mrlbm_batch_autotune_walls(device, q, storage, len(shapes), sh, len(dims), dm, wl,
+                                                          steps, ctypes.byref(best), ms.ctypes.data_as(_lib._dp)))
     return tuple(shapes[best.value]), {tuple(s): (None if m < 0 else float(m)) for s, m in zip(shapes, ms)}

@@ -115,6 +115,9 @@ _SIGS = {
     "mrlbm_batch_destroy": [_vp],
     "mrlbm_batch_autotune": [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                              ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp],
+    "mrlbm_batch_autotune_walls": [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int), _dp],
     "mrlbm_classify": [ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_uint8), _u32p],
     "mrlbm_set_moving_wall": [ctypes.c_int] * 6 + [_u32p],
     "mrlbm_voxelize_spheres": [ctypes.c_int] * 4 + [_dp, ctypes.POINTER(ctypes.c_uint8)],

@@ -113,3 +113,17 @@ def test_autotune_picks_fitting_shape():
     say("autotune", best, ms)
     assert ms[(32, 4, 4)] is None and ms[(64, 4, 2)] is None  # 48 is not a multiple of 32 or 64
     assert best in ((8, 8, 8), (16, 8, 4))
+
+
+def test_autotune_with_walls_times_boundary_tiles():
+    """Wall lattices are tuned with device-classified domain walls (the
+    boundary body runs); the choice is a fitting shape and every fitting
+    shape is timed."""
+    dims = [(32, 32, 32), (48, 48, 48)]
+    best, ms = mrlbm.autotune(dims, steps=20, walls=[(1, 1, 1), (0, 1, 1)])
+    say("autotune walls", best, ms)
+    assert ms[(32, 4, 4)] is None and ms[(64, 4, 2)] is None
+    assert best in ((8, 8, 8), (16, 8, 4))
+    assert all(ms[s] > 0 for s in ((8, 8, 8), (16, 8, 4)))
+    with pytest.raises(ValueError):
+        mrlbm.autotune(dims, steps=20, walls=[(1, 1, 1)])
