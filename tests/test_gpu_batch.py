@@ -45,7 +45,7 @@ def test_batch_equals_alone(mode):
         for lat in lats:
             b.add(lat)
         if mode == "persistent":
-            b.set_grid(300)
+            b.set_grid(100)  # below both launches' item counts (296 bulk, 568 boundary): ranged kernel
         b.step(steps, graph_steps=10 if mode == "graph" else 0)
         for lat, r in zip(lats, ref):
             assert lat.info()["step"] == steps
@@ -54,13 +54,15 @@ def test_batch_equals_alone(mode):
             lat.close()
 
 
-def test_batch_fp16_equals_alone():
+@pytest.mark.parametrize("grid", [0, 100])
+def test_batch_fp16_equals_alone(grid):
     steps = 30
     ref = alone(steps, mrlbm.STORE_FP16)
     with mrlbm.Batch(q=19, tile=(8, 8, 8), storage=mrlbm.STORE_FP16) as b:
         lats = [make(n, tau, u, mrlbm.STORE_FP16, seed=i) for i, (n, tau, u) in enumerate(CASES)]
         for lat in lats:
             b.add(lat)
+        b.set_grid(grid)
         b.step(steps, graph_steps=10)
         for lat, r in zip(lats, ref):
             assert np.array_equal(lat.moments(), r)
