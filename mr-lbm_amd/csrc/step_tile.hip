@@ -3,19 +3,21 @@
 // Same data flow as the reference's gpuMomCollisionStream
 // (src/mlbm.cu:3-714), written for gfx950:
 //   1. load the 10 stored moments (block-major SoA: each wave64 reads
-//      contiguous 256-byte runs per moment plane)
+//      contiguous runs per moment plane; non-temporal)
 //   2. reconstruct the post-collision populations of the previous step
 //      (as deviations g_i = f_i - w_i RHO_0, see lattice.h)
-//   3. write populations 1..Q-1 to LDS arrays with a one-cell halo on the
-//      upwind side; whole waves copy the neighbours' ghost planes into the
-//      halo (halo.h; the X, Y, Z winner rule of
-//      src/includeFiles/popLoad.inc:2-86); barrier; every node pulls its
-//      populations at compile-time offsets from its own base address
-//   5. boundary class -> moment-based BC, else bulk moments; scale; collide
-//   6. store moments in place, reconstruct, stage the outgoing populations
-//      in LDS and let whole waves write the ghost planes in contiguous runs
-//      (src/includeFiles/popSave.inc:2-78 with the wall predicates of
-//      src/includeFiles/interface.h:6-47)
+//   3. write populations 1..Q-1 to per-population LDS arrays with the
+//      entering plane on the upwind side (x-only movers: DPP lane shifts);
+//      whole waves copy the neighbours' ghost planes into those planes
+//      (halo.h; the X, Y, Z winner rule of src/includeFiles/popLoad.inc:2-86);
+//      barrier; every node pulls its populations at compile-time offsets
+//      from its base address
+//   4. boundary class -> moment-based BC, else bulk moments; scale; collide;
+//      round to the storage format
+//   5. store moments in place; stage each node's reconstruction terms in
+//      LDS and let whole waves evaluate and write the outgoing ghost
+//      populations in contiguous runs (src/includeFiles/popSave.inc:2-78
+//      with the wall predicates of src/includeFiles/interface.h:6-47)
 // Solid nodes run the same code (no early exit before a barrier) and store
 // nothing.
 #include "halo.h"
