@@ -254,6 +254,29 @@ def test_reduced_precision_taylor_green(storage, bar):
     assert abs(gk / ok - 1) <= bar
 
 
+@pytest.mark.parametrize("storage", [mrlbm.STORE_FP32, mrlbm.STORE_FP16])
+def test_d3q27_walls_tile_shapes_bit_identical(storage):
+    """D3Q27 cavity (boundary body, edge and corner classes): tiles whose x
+    rows move the x-only populations by lane shifts (8, 16) and one that
+    streams them through LDS (32) give the same bits."""
+    n = 32
+    types = geometry.cavity(n)
+    ref = None
+    for tile in [(8, 8, 8), (16, 8, 4), (32, 4, 4)]:
+        say("d3q27 tile", tile, storage)
+        with mrlbm.Lattice((n, n, n), q=27, tau=0.6, walls=(1, 1, 1), tile=tile, storage=storage, seed=5) as lat:
+            lat.set_bc_velocity(0, (0.05, 0, 0))
+            lat.set_node_types(types)
+            lat.init_uniform()
+            lat.step(50)
+            m = lat.moments()
+        assert np.isfinite(m).all()
+        if ref is None:
+            ref = m
+        else:
+            assert np.array_equal(m, ref), tile
+
+
 @pytest.mark.parametrize("storage", [mrlbm.STORE_FP16, mrlbm.STORE_BF16])
 def test_reduced_precision_tile_independent(storage):
     """Ghost populations come from the rounded moments, and the rounding is
