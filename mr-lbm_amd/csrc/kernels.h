@@ -143,7 +143,7 @@ struct InstArgs {
     int pad;
 };
 
-// Batched step (step_tile.hip): one launch advances every lattice of the
+// Batched step (step_tile_body.h): one launch advances every lattice of the
 // work table by one step. work[i] = instance << 20 | tile. grid <= 0 or
 // grid >= nwork: one item per block; otherwise grid blocks, block b taking
 // the contiguous items [b*per, (b+1)*per), per = ceil(nwork/grid), with the
@@ -153,7 +153,8 @@ hipError_t launch_step_batch(int q, int bx, int by, int bz, int storage, bool ha
                              const uint32_t* work, int nwork, int grid, int k, hipStream_t s);
 hipError_t launch_batch_advance(InstArgs* tab, int ninst, int nsteps, hipStream_t s);
 
-// Launchers (step_tile.hip). Return hipError_t of the launch.
+// Launchers (step_tile.hip, dispatching to the per-storage translation units
+// of step_tile_body.h). Return hipError_t of the launch.
 // Step of the n tiles list[0..n) (tiles 0..n-1 when list is null), one
 // workgroup per tile. has_bc = false only for tiles whose nodes are all
 // bulk.
