@@ -181,54 +181,56 @@ def serving_ducts(n_req=100, rate=20.0, seed=5, max_batch=8):
     }
 
 
-def reduced_precision_variants(n, tau, u0, steps, warmup, fp32_mlups):
-    """Same workload with FP16 and BF16 moment storage (FP32 arithmetic);
-    each run's fields are compared with an FP32-storage run of the same
-    step count (per-field relative L2 of rho - 1 and u)."""
+def launch_windows(lat, launches, windows=3):
+    """Mean launch time of `lat`'s step in seconds: the median over `windows`
+    windows of `launches` back-to-back launches, each timed with HIP events
+    on the lattice's stream."""
+    per = sorted(lat.time_steps(launches) * 1e-3 / launches for _ in range(windows))
+    return per[len(per) // 2], per
+
+
+def reduced_precision_variants(n, tau, u0, launches, warmup):
+    """Same workload with FP32, FP16 and BF16 moment storage (FP32 arithmetic
+    in all three), each timed the same way: `warmup` launches, then the
+    median of three event-timed windows of `launches` launches. The speedups
+    divide matched windows. Each reduced-precision run's fields are compared
+    with the FP32-storage run after the same number of steps (per-field
+    relative L2 of rho - 1, rho and u)."""
     import numpy as np
     import mrlbm
 
-    def fields(st):
+    res, ref = {}, None
+    for name, st in (("fp32", mrlbm.STORE_FP32), ("fp16", mrlbm.STORE_FP16), ("bf16", mrlbm.STORE_BF16)):
         lat = mrlbm.Lattice((n, n, n), q=19, tau=tau, storage=st)
         lat.init_taylor_green(u0)
-        return lat
-
-    ref = fields(mrlbm.STORE_FP32)
-    ref.step(warmup + steps)
-    rho_r, ux_r, uy_r, uz_r = ref.macroscopic()
-    ref.close()
-    res = {}
-    for name, st in (("fp16", mrlbm.STORE_FP16), ("bf16", mrlbm.STORE_BF16)):
-        lat = fields(st)
         info = lat.info()
         lat.step(warmup)
         lat.synchronize()
-        t0 = time.perf_counter()
-        ev_ms = lat.time_steps(steps)
-        lat.synchronize()
-        t1 = time.perf_counter()
+        avg, per = launch_windows(lat, launches)
         rho, ux, uy, uz = lat.macroscopic()
         lat.close()
-        avg = ev_ms * 1e-3 / steps
-        mlups = n ** 3 * steps / (t1 - t0) / 1e6
         achieved = info["bytes_per_lu"] * n ** 3 / avg / 1e9
-
-        def rl2(a, b):
-            return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
-
         res[name] = {
-            "value": round(mlups, 1),
+            "value": round(n ** 3 / avg / 1e6, 1),
             "unit": "MLUPS",
-            "steps": steps,
-            "speedup_vs_fp32": round(mlups / fp32_mlups, 3),
+            "timing": f"median of 3 windows of {launches} launches after {warmup} warm-up launches (HIP events)",
+            "window_launch_us": [round(t * 1e6, 2) for t in per],
             "tile": [info["bx"], info["by"], info["bz"]],
             "bytes_per_lu": info["bytes_per_lu"],
             "mean_launch_us": round(avg * 1e6, 2),
             "achieved_gbs": round(achieved, 1),
             "roofline_frac": round(achieved / PEAK_HBM_GBS, 4),
-            "rel_l2_vs_fp32_storage": {"rho": rl2(rho, rho_r), "rho-1": rl2(rho - 1.0, rho_r - 1.0),
-                                       "u": rl2(np.stack([ux, uy, uz]), np.stack([ux_r, uy_r, uz_r]))},
         }
+        if ref is None:
+            ref = (avg, rho, np.stack([ux, uy, uz]))
+            continue
+
+        def rl2(a, b):
+            return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+        res[name]["speedup_vs_fp32"] = round(ref[0] / avg, 3)
+        res[name]["rel_l2_vs_fp32_storage"] = {"rho-1": rl2(rho - 1.0, ref[1] - 1.0), "rho": rl2(rho, ref[1]),
+                                               "u": rl2(np.stack([ux, uy, uz]), ref[2])}
     return res
 
 
@@ -315,6 +317,11 @@ def main():
     info = lat.info()
     tile = (info["bx"], info["by"], info["bz"])
     kname = {mrlbm.KERNEL_TILE: "tile", mrlbm.KERNEL_COLUMN: "column"}[info["kernel"]]
+    # roofline of the step kernel: median of three event-timed windows of at
+    # least 200 launches (untimed by the headline), then the headline's own
+    # W warm-up steps and exactly K timed steps
+    roof_launches = max(200, args.steps)
+    avg_launch_s, roof_windows = launch_windows(lat, roof_launches)
     lat.step(args.warmup)
     lat.synchronize()
     t0 = time.perf_counter()
@@ -328,7 +335,6 @@ def main():
 
     wall = t1 - t0
     mlups = N * args.steps / wall / 1e6
-    avg_launch_s = ev_ms * 1e-3 / args.steps
     b_lu = info["bytes_per_lu"]
     achieved = b_lu * N / avg_launch_s / 1e9
     key = f"tgv{n}_d3q19_{args.storage}_{kname}{tile[0]}x{tile[1]}x{tile[2]}"
@@ -358,6 +364,9 @@ def main():
             "vs_baseline_ref": "A100 11390 MLUPS, D3Q19 FP32 256^3 (reference README.md:141)",
             "copy_kernel_gbs": round(copy_gbs, 1),
             "mean_launch_us": round(avg_launch_s * 1e6, 2),
+            "roofline_timing": f"median of 3 HIP-event windows of {roof_launches} launches, before the timed steps",
+            "roofline_window_launch_us": [round(t * 1e6, 2) for t in roof_windows],
+            "timed_steps_event_launch_us": round(ev_ms * 1e3 / args.steps, 2),
             "final_mean_ke": ke,
         },
         "roofline": {
@@ -381,7 +390,7 @@ def main():
         out["serving"] = serving_ducts()
     if not args.no_variants:
         phase("FP16/BF16 storage")
-        out["variants"] = reduced_precision_variants(n, tau, u0, args.steps // 2, args.warmup, mlups)
+        out["variants"] = reduced_precision_variants(n, tau, u0, max(200, args.steps), max(20, args.warmup))
     if not args.no_channel:
         phase("D3Q27 duct at FP32/FP16/BF16")
         out["channel_d3q27"] = channel_d3q27(args.channel_steps)

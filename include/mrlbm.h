@@ -172,15 +172,46 @@ int mrlbm_mean_flow_read(mrlbm_ctx* ctx, double* out4n, int64_t* count);
  * Replaces turbulentKineticEnergy (src/treatData.cu:236-296). */
 int mrlbm_turbulent_ke(mrlbm_ctx* ctx, double* tke);
 
-/* Checkpoint = moments + current ghost set + step + parameters, raw
- * little-endian, versioned header. Replaces saveSimCheckpoint /
- * loadSimCheckpoint (src/checkpoint.cu:372-382, :353-369). */
+/* Checkpoint = parameters + step + stored moments + the ghost set read
+ * next, one little-endian file. Replaces saveSimCheckpoint /
+ * loadSimCheckpoint / operateSimCheckpoint (src/checkpoint.cu:372-382,
+ * :353-369, :129-316), whose state is the same (fMom, fGhost, step).
+ * Layout: a MRLBM_CHECKPOINT_HEADER_BYTES header
+ *   char magic[8] = "MRLBMCK2"; uint32 version = 2;
+ *   int32 nx, ny, nz, q, bx, by, bz, storage, wall_x, wall_y, wall_z;
+ *   uint32 seed, reserved; int64 step; double tau, rho0, fx, fy, fz;
+ *   uint64 mom_bytes, face_bytes[6];
+ * then mom_bytes of moments (block layout, storage words: FP16/BF16 state
+ * is saved bit-exactly) and face_bytes[f] of ghost face f (FP32, f_i -
+ * w_i RHO_0). Load rejects (MRLBM_EIO) a file whose sizes do not follow
+ * from its lattice or whose length is not header + payloads, and
+ * (MRLBM_EINVAL) one whose lattice, tile, storage, walls, RHO_0, tau,
+ * force or seed differ from the context's; it reads the whole file before
+ * it changes the device state. Node types are not part of the state: set
+ * them on the context as before the save. A lattice stepped in a batch
+ * may be saved and loaded between batch steps. */
+#define MRLBM_CHECKPOINT_HEADER_BYTES 168
 int mrlbm_checkpoint_save(mrlbm_ctx* ctx, const char* path);
 int mrlbm_checkpoint_load(mrlbm_ctx* ctx, const char* path);
+/* Host only: validate a checkpoint file's structure and return the
+ * parameters it was saved with (kernel = AUTO, device = 0) and its step. */
+int mrlbm_checkpoint_info(const char* path, mrlbm_params* out, int64_t* step);
 
 /* Legacy-VTK STRUCTURED_POINTS file (big-endian float rho, velocity),
  * as saveVarVTK (src/saveData.cu:303-406). */
 int mrlbm_save_vtk(mrlbm_ctx* ctx, const char* path);
+
+/* Raw field files for the reference's post-processing tools
+ * (post/fileTreat.py): dir/ID_rho<step>.bin, _ux, _uy, _uz, float32,
+ * x-fastest, step zero-padded as getVarFilename does; rho = RHO_0 +
+ * stored, u = the stored 3u, as saveMacr + saveVarBin write them
+ * (src/saveData.cu:65-70, :183-228, :262-282). The directory must exist. */
+int mrlbm_save_bin(mrlbm_ctx* ctx, const char* dir, const char* id);
+/* dir/ID_info.txt in the reference's layout (getSimInfoString/saveSimInfo,
+ * src/saveData.cu:469-613): lattice, tau, u_max (the case's U_MAX), force,
+ * save interval, current step, MLUPS, tile, case name. */
+int mrlbm_save_info(mrlbm_ctx* ctx, const char* dir, const char* id, const char* case_name, double u_max,
+                    int64_t save_steps, double mlups);
 
 int mrlbm_get_info(mrlbm_ctx* ctx, mrlbm_info* out);
 

@@ -543,136 +543,4 @@ int mrlbm_bc_class_info(int q, uint32_t node_type, double omega, const double u[
     return MRLBM_OK;
 }
 
-// ---------------------------------------------------------------------------
-// checkpoint: header, then moments (block layout, storage type), then the
-// six faces of the ghost set the next step reads. Ghost values are the
-// populations as the kernels carry them, f_i - w_i RHO_0.
-
-struct CkHeader {
-    char magic[8];
-    uint32_t version;
-    int32_t nx, ny, nz, q, bx, by, bz, storage, wall_x, wall_y, wall_z, pad;
-    int64_t step;
-    double tau, rho0, fx, fy, fz;
-    uint64_t mom_bytes, face_bytes[6];
-};
-
-int mrlbm_checkpoint_save(mrlbm_ctx* c, const char* path) {
-    if (!c || !path) return fail(MRLBM_EINVAL, "null argument");
-    if (!c->initialised) return fail(MRLBM_ESTATE, "lattice not initialised");
-    HIPCHK(hipSetDevice(c->p.device));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    CkHeader h{};
-    std::memcpy(h.magic, "MRLBMCK1", 8);
-    h.version = 1;
-    h.nx = c->p.nx;
-    h.ny = c->p.ny;
-    h.nz = c->p.nz;
-    h.q = c->p.q;
-    h.bx = c->bx;
-    h.by = c->by;
-    h.bz = c->bz;
-    h.storage = c->p.storage;
-    h.wall_x = c->p.wall_x;
-    h.wall_y = c->p.wall_y;
-    h.wall_z = c->p.wall_z;
-    h.step = c->step;
-    h.tau = c->p.tau;
-    h.rho0 = c->p.rho0;
-    h.fx = c->p.fx;
-    h.fy = c->p.fy;
-    h.fz = c->p.fz;
-    h.mom_bytes = c->N * 10 * storage_bytes(c->p.storage);
-    for (int f = 0; f < 6; f++) h.face_bytes[f] = c->face_elems[f] * sizeof(float);
-    FILE* fp = std::fopen(path, "wb");
-    if (!fp) return fail(MRLBM_EIO, "cannot open '%s' for writing", path);
-    std::vector<char> buf(h.mom_bytes);
-    bool ok = std::fwrite(&h, sizeof h, 1, fp) == 1;
-    if (ok && hipMemcpy(buf.data(), c->d_mom, h.mom_bytes, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
-    if (ok) ok = std::fwrite(buf.data(), 1, h.mom_bytes, fp) == h.mom_bytes;
-    for (int f = 0; f < 6 && ok; f++) {
-        buf.resize(h.face_bytes[f]);
-        if (hipMemcpy(buf.data(), c->d_ghost[c->cur][f], h.face_bytes[f], hipMemcpyDeviceToHost) != hipSuccess)
-            ok = false;
-        if (ok) ok = std::fwrite(buf.data(), 1, h.face_bytes[f], fp) == h.face_bytes[f];
-    }
-    if (std::fclose(fp) != 0) ok = false;
-    if (!ok) return fail(MRLBM_EIO, "writing checkpoint '%s' failed", path);
-    return MRLBM_OK;
-}
-
-int mrlbm_checkpoint_load(mrlbm_ctx* c, const char* path) {
-    if (!c || !path) return fail(MRLBM_EINVAL, "null argument");
-    HIPCHK(hipSetDevice(c->p.device));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    FILE* fp = std::fopen(path, "rb");
-    if (!fp) return fail(MRLBM_EIO, "cannot open '%s'", path);
-    CkHeader h{};
-    if (std::fread(&h, sizeof h, 1, fp) != 1 || std::memcmp(h.magic, "MRLBMCK1", 8) != 0 || h.version != 1) {
-        std::fclose(fp);
-        return fail(MRLBM_EIO, "'%s' is not a version-1 checkpoint", path);
-    }
-    if (h.nx != c->p.nx || h.ny != c->p.ny || h.nz != c->p.nz || h.q != c->p.q || h.bx != c->bx || h.by != c->by ||
-        h.bz != c->bz || h.storage != c->p.storage) {
-        std::fclose(fp);
-        return fail(MRLBM_EINVAL, "checkpoint lattice/tile/storage does not match the context");
-    }
-    std::vector<char> buf(h.mom_bytes);
-    bool ok = std::fread(buf.data(), 1, h.mom_bytes, fp) == h.mom_bytes;
-    if (ok && hipMemcpy(c->d_mom, buf.data(), h.mom_bytes, hipMemcpyHostToDevice) != hipSuccess) ok = false;
-    for (int f = 0; f < 6 && ok; f++) {
-        if (h.face_bytes[f] != c->face_elems[f] * sizeof(float)) {
-            ok = false;
-            break;
-        }
-        buf.resize(h.face_bytes[f]);
-        ok = std::fread(buf.data(), 1, h.face_bytes[f], fp) == h.face_bytes[f];
-        // both sets: the one read next and the one written next (boundary
-        // slots that are never rewritten keep their saved contents)
-        for (int s = 0; s < 2 && ok; s++)
-            if (hipMemcpy(c->d_ghost[s][f], buf.data(), h.face_bytes[f], hipMemcpyHostToDevice) != hipSuccess)
-                ok = false;
-    }
-    std::fclose(fp);
-    if (!ok) return fail(MRLBM_EIO, "reading checkpoint '%s' failed", path);
-    c->cur = 0;
-    c->step = h.step;
-    c->initialised = true;
-    c->version++;
-    return MRLBM_OK;
-}
-
-// ---------------------------------------------------------------------------
-// legacy VTK, big-endian float (src/saveData.cu:303-406)
-
-static void put_be(FILE* fp, float v) {
-    uint32_t u;
-    std::memcpy(&u, &v, 4);
-    u = (u >> 24) | ((u >> 8) & 0xFF00u) | ((u << 8) & 0xFF0000u) | (u << 24);
-    std::fwrite(&u, 4, 1, fp);
-}
-
-int mrlbm_save_vtk(mrlbm_ctx* c, const char* path) {
-    if (!c || !path) return fail(MRLBM_EINVAL, "null argument");
-    std::vector<double> m(c->N * 10);
-    int rc = mrlbm_read_moments(c, m.data());
-    if (rc) return rc;
-    FILE* fp = std::fopen(path, "wb");
-    if (!fp) return fail(MRLBM_EIO, "cannot open '%s' for writing", path);
-    std::fprintf(fp,
-                 "# vtk DataFile Version 3.0\nLBM output (binary)\nBINARY\nDATASET STRUCTURED_POINTS\n"
-                 "DIMENSIONS %d %d %d\nORIGIN 0 0 0\nSPACING 1 1 1\nPOINT_DATA %zu\n",
-                 c->p.nx, c->p.ny, c->p.nz, c->N);
-    std::fprintf(fp, "SCALARS rho float 1\nLOOKUP_TABLE default\n");
-    for (size_t j = 0; j < c->N; j++) put_be(fp, (float)(c->p.rho0 + m[j]));
-    std::fprintf(fp, "VECTORS velocity float\n");
-    for (size_t j = 0; j < c->N; j++) {
-        put_be(fp, (float)(m[c->N + j] / 3.0));
-        put_be(fp, (float)(m[2 * c->N + j] / 3.0));
-        put_be(fp, (float)(m[3 * c->N + j] / 3.0));
-    }
-    if (std::fclose(fp) != 0) return fail(MRLBM_EIO, "writing '%s' failed", path);
-    return MRLBM_OK;
-}
-
 }  // extern "C"

@@ -180,10 +180,29 @@ class Lattice:
     def save_vtk(self, path):
         _lib.check(_lib.load().mrlbm_save_vtk(self._h, str(path).encode()))
 
+    def save_bin(self, directory, sim_id):
+        """directory/ID_{rho,ux,uy,uz}<step>.bin as the reference's saveMacr writes them."""
+        _lib.check(_lib.load().mrlbm_save_bin(self._h, str(directory).encode(), str(sim_id).encode()))
+
+    def save_info(self, directory, sim_id, case_name="", u_max=0.0, save_steps=0, mlups=0.0):
+        """directory/ID_info.txt in the reference's layout."""
+        _lib.check(_lib.load().mrlbm_save_info(self._h, str(directory).encode(), str(sim_id).encode(),
+                                               str(case_name).encode(), float(u_max), int(save_steps),
+                                               float(mlups)))
+
     def info(self):
         i = _lib.Info()
         _lib.check(_lib.load().mrlbm_get_info(self._h, ctypes.byref(i)))
         return {k: getattr(i, k) for k, _ in _lib.Info._fields_}
+
+
+def checkpoint_info(path):
+    """(parameters dict, step) a checkpoint file was saved with; validates
+    the file's structure on the host (no device needed)."""
+    p = _lib.Params()
+    step = ctypes.c_int64()
+    _lib.check(_lib.load().mrlbm_checkpoint_info(str(path).encode(), ctypes.byref(p), ctypes.byref(step)))
+    return {k: getattr(p, k) for k, _ in _lib.Params._fields_}, step.value
 
 
 def copy_bandwidth(nbytes=1 << 30, iters=20, device=0):

@@ -100,6 +100,10 @@ _SIGS = {
     "mrlbm_checkpoint_save": [_vp, ctypes.c_char_p],
     "mrlbm_checkpoint_load": [_vp, ctypes.c_char_p],
     "mrlbm_save_vtk": [_vp, ctypes.c_char_p],
+    "mrlbm_checkpoint_info": [ctypes.c_char_p, ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_int64)],
+    "mrlbm_save_bin": [_vp, ctypes.c_char_p, ctypes.c_char_p],
+    "mrlbm_save_info": [_vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_int64,
+                        ctypes.c_double],
     "mrlbm_get_info": [_vp, ctypes.POINTER(Info)],
     "mrlbm_last_error": [],
     "mrlbm_bc_class_info": [ctypes.c_int, ctypes.c_uint32, ctypes.c_double, _dp, _u32p, _dp],

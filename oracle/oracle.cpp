@@ -233,6 +233,8 @@ struct orc {
     std::vector<double> fpost;  // q * N post-collision populations
     std::vector<uint32_t> type;  // node types (0 = bulk)
     bool any_bc = false;
+    int gtile[3] = {0, 0, 0};     // reference ghost semantics for this tile shape (0: off)
+    std::vector<double> mom0;     // moments the ghost planes were built from (ghost semantics only)
 };
 
 extern "C" {
@@ -319,6 +321,7 @@ static int init_any(orc* o, int mode, const double* rho, const double* ux, const
                 init_node(o->q, mode, o->rho0, r, a, b, c, m);
                 for (int k = 0; k < 10; k++) o->mom[(size_t)k * o->N + j] = m[k];
             }
+    if (o->gtile[0]) o->mom0 = o->mom;
     return 0;
 }
 
@@ -331,6 +334,40 @@ int orc_init_taylor_green(orc* o, double u0, int mode) {
 int orc_init_uniform(orc* o, double rho, double ux, double uy, double uz, int mode) {
     const double u[4] = {rho, ux, uy, uz};
     return init_any(o, mode, nullptr, nullptr, nullptr, nullptr, false, 0.0, u);
+}
+
+// A population whose source lies outside the domain along a wall axis,
+// read as the reference's kernel reads it (src/includeFiles/popLoad.inc:2-86
+// with the wall predicates of src/includeFiles/interface.h:6-47), for the
+// tile shape o->gtile: from the ghost plane of the face that wins in
+// popLoad.inc - the last axis, in X, Y, Z order, along which the source
+// leaves the puller's tile - of the tile holding the periodically wrapped
+// source (sx, sy, sz). A face on a wall-axis domain boundary is written
+// only at initialisation (gpuInitialization_pop, then gGhost <- fGhost,
+// src/main.cuh:1016) and holds the source's initial population; any other
+// face holds its current post-collision population.
+static double ghost_value(const orc* o, int i, int x, int y, int z, int sx, int sy, int sz) {
+    const int c[3] = {CX[i], CY[i], CZ[i]};
+    const int p[3] = {x, y, z}, s[3] = {sx, sy, sz}, n[3] = {o->nx, o->ny, o->nz};
+    int axis = -1;
+    for (int a = 0; a < 3; a++) {
+        const int l = p[a] % o->gtile[a] - c[a];
+        if (l < 0 || l >= o->gtile[a]) axis = a;
+    }
+    const size_t N = o->N;
+    const size_t src = sx + (size_t)o->nx * (sy + (size_t)o->ny * sz);
+    bool stale = false;
+    if (axis >= 0 && o->wall[axis]) {
+        const int tile = s[axis] / o->gtile[axis], ntile = n[axis] / o->gtile[axis];
+        stale = c[axis] > 0 ? tile == ntile - 1 : tile == 0;
+    }
+    if (!stale) return o->fpost[(size_t)i * N + src];
+    if ((o->type[src] & 0xFFu) == 0xFFu) return NaN;
+    double m[10], f[27];
+    for (int k = 0; k < 10; k++) m[k] = o->mom0[(size_t)k * N + src];
+    m[0] += o->rho0;
+    reconstruct(o->q, m, f);
+    return f[i];
 }
 
 static void one_step(orc* o) {
@@ -380,7 +417,14 @@ static void one_step(orc* o) {
                         if (o->wall[2]) out = true;
                         sz = (sz + nz) % nz;
                     }
-                    f[i] = out ? NaN : fp[(size_t)i * N + sx + (size_t)nx * (sy + (size_t)ny * sz)];
+                    const size_t src = sx + (size_t)nx * (sy + (size_t)ny * sz);
+                    if (!out) {
+                        f[i] = fp[(size_t)i * N + src];
+                    } else if (!o->gtile[0]) {
+                        f[i] = NaN;
+                    } else {
+                        f[i] = ghost_value(o, i, x, y, z, sx, sy, sz);
+                    }
                 }
                 double m[10];
                 if (t == 0u) {
@@ -431,7 +475,24 @@ double orc_step_timed(orc* o, int nsteps) {
 }
 
 void orc_get_moments(const orc* o, double* out) { std::memcpy(out, o->mom.data(), o->mom.size() * sizeof(double)); }
-void orc_set_moments(orc* o, const double* in) { std::memcpy(o->mom.data(), in, o->mom.size() * sizeof(double)); }
+void orc_set_moments(orc* o, const double* in) {
+    std::memcpy(o->mom.data(), in, o->mom.size() * sizeof(double));
+    if (o->gtile[0]) o->mom0 = o->mom;
+}
+
+int orc_set_ghost_tiles(orc* o, int bx, int by, int bz) {
+    if (bx == 0 && by == 0 && bz == 0) {
+        o->gtile[0] = o->gtile[1] = o->gtile[2] = 0;
+        o->mom0.clear();
+        return 0;
+    }
+    if (bx <= 0 || by <= 0 || bz <= 0 || o->nx % bx || o->ny % by || o->nz % bz) return -1;
+    o->gtile[0] = bx;
+    o->gtile[1] = by;
+    o->gtile[2] = bz;
+    o->mom0 = o->mom;
+    return 0;
+}
 
 void orc_reduce(const orc* o, double* mass, double* ke) {
     long double sm = 0, sk = 0;

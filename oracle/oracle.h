@@ -48,7 +48,19 @@ int orc_step(orc* o, int nsteps);
 /* seconds of steady_clock time spent in nsteps (the step loop only) */
 double orc_step_timed(orc* o, int nsteps);
 void orc_get_moments(const orc* o, double* out10n);
+/* also the moments the ghost planes are rebuilt from (ghost semantics) */
 void orc_set_moments(orc* o, const double* in10n);
+/* Reference ghost semantics for tile shape bx x by x bz (0,0,0: off, the
+ * default). Off, a population streamed from outside a wall axis is NaN, so
+ * a boundary rule that reads one poisons its node. On, it is the value the
+ * reference's kernel reads (src/includeFiles/popLoad.inc:2-86,
+ * interface.h:6-47): the ghost plane of the face winning in popLoad.inc's
+ * X, Y, Z order, of the tile holding the periodically wrapped source;
+ * never rewritten after initialisation when that face is on a wall-axis
+ * domain boundary (initial population), else the wrapped source's current
+ * one. Call before the initialisation. Returns 0, or -1 if the tile does
+ * not divide the lattice. */
+int orc_set_ghost_tiles(orc* o, int bx, int by, int bz);
 /* mean(rho - RHO_0) and mean(rho |u|^2 / 2), all nodes, long double sums */
 void orc_reduce(const orc* o, double* mass, double* ke);
 /* number of NaN values in the stored moments of non-solid nodes */

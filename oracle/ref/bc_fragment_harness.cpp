@@ -2,8 +2,8 @@
 // single node, in FP64 (dfloat = double), through the fragment's own
 // variable contract (src/mlbm.cu:456-461): in pop[Q], nodeType, omegaVar,
 // U_MAX; out rhoVar, ux_t30..uz_t30, m_xx_t45..m_zz_t45 (unscaled).
-// The node-type macros come from this project's include/mrlbm_nodetype.h,
-// the header case fragments are written against.
+// The node-type macros are the reference's own: the #define lines of
+// src/nodeTypeMap.h, extracted by oracle/ref/Makefile (NODETYPE_HEADER).
 //
 // TEST INFRASTRUCTURE ONLY: built by oracle/ref/Makefile into oracle/_ref/
 // from the fragment files where they lie under the reference tree, and used
@@ -11,7 +11,10 @@
 #include <cmath>
 #include <limits>
 
-#include "../../include/mrlbm_nodetype.h"
+#ifndef NODETYPE_HEADER
+#error "NODETYPE_HEADER must name the node-type constants extracted from src/nodeTypeMap.h"
+#endif
+#include NODETYPE_HEADER
 
 #ifndef BC_Q
 #define BC_Q 19

@@ -35,6 +35,7 @@ def lib():
         L.orc_step_timed.restype = ctypes.c_double
         L.orc_get_moments.argtypes = [ctypes.c_void_p, _dp]
         L.orc_set_moments.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_set_ghost_tiles.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_reduce.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_count_nan.argtypes = [ctypes.c_void_p]
         L.orc_count_nan.restype = ctypes.c_int64
@@ -78,6 +79,10 @@ class Oracle:
             return
         t = np.ascontiguousarray(types.reshape(-1, order="F") if types.ndim == 3 else types, dtype=np.uint32)
         assert lib().orc_set_node_types(self.h, t.ctypes.data_as(_u32p)) == 0
+
+    def set_ghost_tiles(self, tile):
+        """Reference ghost semantics for this tile shape (None: off)."""
+        assert lib().orc_set_ghost_tiles(self.h, *(tile or (0, 0, 0))) == 0
 
     def init_taylor_green(self, u0, mode=0):
         assert lib().orc_init_taylor_green(self.h, u0, mode) == 0

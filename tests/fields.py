@@ -2,13 +2,19 @@
 import numpy as np
 
 
-def rel_l2(a, b):
+def rel_l2(a, b, rho0=1.0):
     """Per-field-group relative L2 errors of two (10, N) moment arrays in the
-    stored convention: density (rho0 + m0), velocity (3 components) and
-    second moments (6 components)."""
+    stored convention:
+      drho  the stored density field rho - rho0 (plane 0) itself, so that an
+            error in the density deviation is measured against the deviation
+            and not against rho ~ 1;
+      rho   the full density rho0 + plane 0;
+      u     the velocity (planes 1-3);
+      m     the second moments (planes 4-9)."""
     out = {}
-    rho_a, rho_b = 1.0 + a[0], 1.0 + b[0]
-    out["rho"] = np.linalg.norm(rho_a - rho_b) / np.linalg.norm(rho_b)
+    d0 = a[0] - b[0]
+    out["drho"] = np.linalg.norm(d0) / max(np.linalg.norm(b[0]), 1e-300)
+    out["rho"] = np.linalg.norm(d0) / np.linalg.norm(rho0 + b[0])
     du, ub = a[1:4] - b[1:4], b[1:4]
     out["u"] = np.linalg.norm(du) / max(np.linalg.norm(ub), 1e-300)
     dm, mb = a[4:10] - b[4:10], b[4:10]

@@ -4,7 +4,9 @@ Runs only in a container that holds the reference tree: it loads the
 fragment libraries that oracle/ref/Makefile compiles into oracle/_ref/ from
 the reference's own cases/*/bc_definition.inc files, evaluates each node
 type on seeded random populations and writes tests/golden/bc_<case>.npz
-(inputs and outputs only). The tests read the .npz files; they never need
+(inputs and outputs only), and records the reference's node-type constants
+(src/nodeTypeMap.h, compiled into libnodetype_ref.so) as
+tests/golden/nodetype_ref.npz. The tests read the .npz files; they never need
 the reference or oracle/_ref.
 
     make -C oracle/ref && python tests/golden/gen_bc_vectors.py
@@ -22,15 +24,21 @@ CX = [0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1
 CY = [0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1]
 CZ = [0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1]
 
-# node-type words named in the wall fragments (include/mrlbm_nodetype.h)
-NAMED = {
-    "NORTH": 0xCC, "SOUTH": 0x33, "WEST": 0x55, "EAST": 0xAA, "FRONT": 0xF0, "BACK": 0x0F,
-    "NORTH_WEST": 0xDD, "NORTH_EAST": 0xEE, "NORTH_FRONT": 0xFC, "NORTH_BACK": 0xCF,
-    "SOUTH_WEST": 0x77, "SOUTH_EAST": 0xBB, "SOUTH_FRONT": 0xF3, "SOUTH_BACK": 0x3F,
-    "WEST_FRONT": 0xF5, "WEST_BACK": 0x5F, "EAST_FRONT": 0xFA, "EAST_BACK": 0xAF,
-    "NORTH_WEST_FRONT": 0xFD, "NORTH_WEST_BACK": 0xDF, "NORTH_EAST_FRONT": 0xFE, "NORTH_EAST_BACK": 0xEF,
-    "SOUTH_WEST_FRONT": 0xF7, "SOUTH_WEST_BACK": 0x7F, "SOUTH_EAST_FRONT": 0xFB, "SOUTH_EAST_BACK": 0xBF,
-}
+# wall node types named in the fragments: faces, edges, corners
+WALL_NAMES = ["NORTH", "SOUTH", "WEST", "EAST", "FRONT", "BACK",
+              "NORTH_WEST", "NORTH_EAST", "NORTH_FRONT", "NORTH_BACK", "SOUTH_WEST", "SOUTH_EAST", "SOUTH_FRONT",
+              "SOUTH_BACK", "WEST_FRONT", "WEST_BACK", "EAST_FRONT", "EAST_BACK",
+              "NORTH_WEST_FRONT", "NORTH_WEST_BACK", "NORTH_EAST_FRONT", "NORTH_EAST_BACK",
+              "SOUTH_WEST_FRONT", "SOUTH_WEST_BACK", "SOUTH_EAST_FRONT", "SOUTH_EAST_BACK"]
+
+
+def reference_node_types():
+    """name -> value of the reference's node-type constants (src/nodeTypeMap.h,
+    compiled by oracle/ref/Makefile into libnodetype_ref.so)."""
+    lib = ctypes.CDLL(os.path.join(REF, "libnodetype_ref.so"))
+    lib.nodetype_ref_name.restype = ctypes.c_char_p
+    lib.nodetype_ref_value.restype = ctypes.c_uint32
+    return {lib.nodetype_ref_name(i).decode(): int(lib.nodetype_ref_value(i)) for i in range(lib.nodetype_ref_count())}
 
 
 def weights(q):
@@ -84,7 +92,12 @@ def run_case(name, types, nsamp, seed, lid=False):
 def main():
     if not os.path.isdir(REF):
         sys.exit("oracle/_ref missing: run make -C oracle/ref first (needs the reference tree)")
-    walls = sorted(NAMED.values())
+    nt = reference_node_types()
+    names = sorted(nt)
+    path = os.path.join(HERE, "nodetype_ref.npz")
+    np.savez_compressed(path, names=np.array(names), values=np.array([nt[k] for k in names], np.uint32))
+    print(f"{path}: {len(names)} node-type constants")
+    walls = sorted(nt[k] for k in WALL_NAMES)
     run_case("lidDrivenCavity_3D", walls, 4, 1, lid=True)
     run_case("parallelPlates_D3Q19", [0x33, 0xCC], 8, 2)
     run_case("parallelPlates_D3Q27", [0x33, 0xCC], 8, 3)

@@ -185,15 +185,6 @@ def test_write_read_moments_round_trip():
     assert max(e.values()) <= TOL
 
 
-def test_vtk_written(tmp_path):
-    with mrlbm.Lattice((16, 16, 16), q=19, tau=0.6) as lat:
-        lat.init_taylor_green(0.05)
-        lat.save_vtk(tmp_path / "f.vtk")
-    data = (tmp_path / "f.vtk").read_bytes()
-    assert data.startswith(b"# vtk DataFile Version 3.0")
-    assert len(data) > 16 ** 3 * 16
-
-
 @pytest.mark.parametrize("shape_tile", [((64, 64, 64), (16, 16, 32)), ((64, 32, 128), (32, 8, 64)),
                                         ((64, 64, 64), (64, 4, 64))])
 def test_column_kernel_bit_identical_to_tile_kernel(shape_tile):
@@ -250,7 +241,11 @@ def test_reduced_precision_taylor_green(storage, bar):
     e = rel_l2(g, orc.moments())
     om_, ok = orc.reduce()
     say("storage", storage, "rel L2", e, "ke", gk, ok, "mass", gm, om_)
-    assert e["rho"] <= bar and e["u"] <= bar
+    assert e["rho"] <= bar and e["u"] <= bar and e["m"] <= bar
+    # the stored density deviation (|rho - 1| ~ 5e-4 here) keeps the storage
+    # format's 11 (FP16) / 8 (BF16) significant bits per value; its
+    # stochastic-rounding noise is not damped by viscosity as the velocity's is
+    assert e["drho"] <= 10 * bar
     assert abs(gk / ok - 1) <= bar
 
 
