@@ -150,6 +150,16 @@ int mrlbm_write_moments(mrlbm_ctx* ctx, const double* in10n);
  * Replaces mean_rho / totalKineticEnergy (src/auxFunctions.cu:8-67,
  * src/treatData.cu:105-165, src/reduction.cu:99-155). */
 int mrlbm_reduce(mrlbm_ctx* ctx, double* mean_mass, double* mean_ke);
+/* Same sums, and the number of nodes whose stored rho - RHO_0 or u is not
+ * finite (a NaN or Inf anywhere also makes the sums non-finite). */
+int mrlbm_reduce_checked(mrlbm_ctx* ctx, double* mean_mass, double* mean_ke, int64_t* nonfinite_nodes);
+
+/* FP64 convergence residual of the velocity field: ||u - u_prev|| / ||u||
+ * (L2 over all nodes), u_prev being the velocity at the previous call; the
+ * first call stores the reference (12 B/node, allocated then) and returns
+ * -1. A steady state drives it to zero. No counterpart in the reference,
+ * whose steady-state cases run a fixed N_STEPS. */
+int mrlbm_velocity_residual(mrlbm_ctx* ctx, double* residual);
 
 /* Diagnostics (device-side; replace the reference's treatData family,
  * src/treatData.cu, which copies one node at a time to the host). */
@@ -289,7 +299,10 @@ int mrlbm_read_voxel_csv(const char* path, int nx, int ny, int nz, uint8_t* soli
  * admitted when the HBM budget and max_batch allow, keep their fields
  * resident, advance together one launch per step in rounds of at most
  * quantum_steps (graph-captured when graph_steps divides a full round) and
- * leave exactly at their step budget. */
+ * leave exactly at their step budget. Failures are per request (status <
+ * 0): after every round an FP64 reduction of each member counts non-finite
+ * nodes, and a member with any leaves the batch failed while the others
+ * continue. Percentiles are over the requests that completed. */
 typedef struct {
     double arrival_s;           /* in: arrival time, seconds from start         */
     int nx, ny, nz;             /* in: lattice (multiples of the tile)          */
@@ -304,7 +317,13 @@ typedef struct {
     double latency_s;           /* out: completion - arrival                    */
     double mass0, mass1;        /* out: FP64 mean(rho - 1) before / after       */
     double ke0, ke1;            /* out: FP64 mean kinetic energy before / after */
-    int status;                 /* out: 0 done                                  */
+    int status;                 /* out: 0 done; < 0 failed, an MRLBM_E* code:
+                                   EINVAL (extents not a multiple of the tile,
+                                   0 or > 8 spheres, no steps, nu <= 0), ENOMEM
+                                   (larger than the whole budget, or allocation
+                                   failed), ENUMERIC (the lattice turned
+                                   non-finite: the body force is not checked,
+                                   so a non-finite one ends here)              */
 } mrlbm_request;
 
 typedef struct {
@@ -324,6 +343,7 @@ typedef struct {
     double p50_s, p99_s, max_s; /* latency percentiles                          */
     double peak_bytes;          /* peak admitted state                          */
     int64_t rounds;
+    int64_t failed;             /* requests that ended with status < 0          */
 } mrlbm_serve_stats;
 
 /* The seeded duct request stream (Poisson arrivals at `rate`/s; Nx in

@@ -70,6 +70,7 @@ int mrlbm_destroy(mrlbm_ctx* c) {
         for (int f = 0; f < 6; f++) hipFree(c->d_ghost[s][f]);
     hipFree(c->d_partial);
     hipFree(c->d_red);
+    hipFree(c->d_uprev);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -171,8 +172,8 @@ int mrlbm_create(const mrlbm_params* pp, mrlbm_ctx** out) {
                 return rc;
             }
     c->red_blocks = 1024;
-    if ((rc = alloc((void**)&c->d_partial, sizeof(double) * 2 * c->red_blocks)) ||
-        (rc = alloc((void**)&c->d_red, sizeof(double) * 2))) {
+    if ((rc = alloc((void**)&c->d_partial, sizeof(double) * 3 * c->red_blocks)) ||
+        (rc = alloc((void**)&c->d_red, sizeof(double) * 4))) {
         mrlbm_destroy(c);
         return rc;
     }
@@ -501,16 +502,45 @@ int mrlbm_write_moments(mrlbm_ctx* c, const double* in) {
     return upload_and_ghosts(c, blk);
 }
 
-int mrlbm_reduce(mrlbm_ctx* c, double* mass, double* ke) {
-    if (!c || !mass || !ke) return fail(MRLBM_EINVAL, "null argument");
+int mrlbm_reduce_checked(mrlbm_ctx* c, double* mass, double* ke, int64_t* nonfinite) {
+    if (!c || !mass || !ke || !nonfinite) return fail(MRLBM_EINVAL, "null argument");
     HIPCHK(hipSetDevice(c->p.device));
     HIPCHK(launch_reduce(c->p.storage, c->d_mom, c->ntiles, c->T, (float)c->p.rho0, c->mrscale[0], c->mrscale[1],
-                         c->d_partial, c->red_blocks, c->d_red, c->stream));
-    double h[2];
+                         c->d_partial, c->red_blocks, c->d_red, c->stream, (double)c->N));
+    double h[3];
     HIPCHK(hipMemcpyAsync(h, c->d_red, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    *mass = h[0] / (double)c->N;
-    *ke = h[1] / (double)c->N;
+    *mass = h[0];
+    *ke = h[1];
+    *nonfinite = (int64_t)h[2];
+    return MRLBM_OK;
+}
+
+int mrlbm_reduce(mrlbm_ctx* c, double* mass, double* ke) {
+    int64_t bad;
+    return mrlbm_reduce_checked(c, mass, ke, &bad);
+}
+
+int mrlbm_velocity_residual(mrlbm_ctx* c, double* residual) {
+    if (!c || !residual) return fail(MRLBM_EINVAL, "null argument");
+    if (!c->initialised) return fail(MRLBM_ESTATE, "lattice not initialised");
+    HIPCHK(hipSetDevice(c->p.device));
+    const bool first = c->d_uprev == nullptr;
+    if (first) {
+        const size_t bytes = sizeof(float) * 3 * c->N;
+        if (hipMalloc((void**)&c->d_uprev, bytes) != hipSuccess) {
+            c->d_uprev = nullptr;
+            return fail(MRLBM_ENOMEM, "hipMalloc(%zu bytes) for the residual reference failed", bytes);
+        }
+        c->device_bytes += (double)bytes;
+        HIPCHK(hipMemsetAsync(c->d_uprev, 0, bytes, c->stream));
+    }
+    HIPCHK(launch_residual(c->p.storage, c->d_mom, c->d_uprev, c->ntiles, c->T, c->mrscale[1], c->d_partial,
+                           c->red_blocks, c->d_red, c->stream));
+    double h[3];
+    HIPCHK(hipMemcpyAsync(h, c->d_red, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *residual = first ? -1.0 : (h[1] > 0.0 ? std::sqrt(h[0] / h[1]) : (h[0] > 0.0 ? INFINITY : 0.0));
     return MRLBM_OK;
 }
 

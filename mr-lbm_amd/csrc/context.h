@@ -56,6 +56,7 @@ struct mrlbm_ctx {
     uint32_t* d_tiles = nullptr;      // all-bulk tiles, then the others (tile kernel with boundaries)
     int n_bulk_tiles = 0;
     double* d_mean = nullptr;         // running sums of rho, ux, uy, uz (4 x N, x-fastest), allocated on first use
+    float* d_uprev = nullptr;         // velocity of the last residual call (3 x N, block layout), on first use
     int64_t mean_count = 0;
 
     size_t mom_index(int x, int y, int z, int m) const {

@@ -175,9 +175,15 @@ hipError_t launch_init_ghosts_column(int q, int bx, int by, int bz, int storage,
 hipError_t launch_init_ghosts(int q, int bx, int by, int bz, int storage, const StepArgs& a, float* const g[6],
                               int ntiles, hipStream_t s);
 
-// FP64 partial sums of (rho - RHO_0) and rho |u|^2 / 2 per block; then one
-// block folds the partials. Deterministic (no atomics).
+// FP64 partial sums of (rho - RHO_0), rho |u|^2 / 2 and the count of
+// non-finite nodes per block (partial: 3 doubles per block); then one block
+// folds the partials into out3, dividing the two sums by n_nodes when it is
+// positive. Deterministic (no atomics).
 hipError_t launch_reduce(int storage, const void* mom, size_t ntiles, int tile_nodes, float rho0, float rs0, float rs1,
-                         double* partial, int nblocks, double* out2, hipStream_t s);
+                         double* partial, int nblocks, double* out3, hipStream_t s, double n_nodes = 0.0);
+// FP64 sums of |u - u_prev|^2 and |u|^2 into out3[0..1] (u_prev: 3 FP32
+// planes of N values, block layout, overwritten with u).
+hipError_t launch_residual(int storage, const void* mom, float* uprev, size_t ntiles, int tile_nodes, float rs1,
+                           double* partial, int nblocks, double* out3, hipStream_t s);
 
 }  // namespace mrlbm

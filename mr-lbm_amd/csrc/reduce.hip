@@ -1,6 +1,7 @@
 // FP64 lattice reductions: sum of (rho - RHO_0) and of rho |u|^2 / 2 over
 // all nodes (solid nodes included, as the reference's sumReductionThread_rho
-// / sumReductionThread_KE do, src/reduction.cu:99-155). The reference sums
+// / sumReductionThread_KE do, src/reduction.cu:99-155), with a count of
+// non-finite nodes, and the velocity convergence residual. The reference sums
 // in FP32 with repeated block passes; here each thread accumulates in FP64,
 // one LDS tree per block, and a single block folds the partials in a fixed
 // order, so the result is deterministic for a given launch shape.
@@ -10,13 +11,16 @@ namespace mrlbm {
 
 constexpr int RED_THREADS = 256;
 
+// Per block: sums of (rho - RHO_0) and rho |u|^2 / 2, and the number of
+// nodes whose rho - RHO_0 or u is not finite (any such node also makes the
+// sums non-finite; the count says how many). partial[3 b + 0..2].
 template <int S>
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_partial(const void* mom_, size_t ntiles, int T, float rho0,
                                                                float rs0, float rs1, double* partial) {
     using C = Codec<S>;
     const typename C::T* mom = (const typename C::T*)mom_;
-    __shared__ double sm[RED_THREADS], sk[RED_THREADS];
-    double am = 0.0, ak = 0.0;
+    __shared__ double sm[RED_THREADS], sk[RED_THREADS], sb[RED_THREADS];
+    double am = 0.0, ak = 0.0, ab = 0.0;
     const size_t N = ntiles * (size_t)T;
     for (size_t j = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; j < N; j += (size_t)gridDim.x * RED_THREADS) {
         const size_t tile = j / T, t = j % T;
@@ -25,48 +29,50 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_partial(const void* mom_
         const double ux = C::ld(mom, b + T, rs1), uy = C::ld(mom, b + 2 * T, rs1), uz = C::ld(mom, b + 3 * T, rs1);
         am += (double)dr;
         ak += ((double)rho0 + (double)dr) * (ux * ux + uy * uy + uz * uz) / 18.0;
+        ab += (__builtin_isfinite(dr) && __builtin_isfinite(ux) && __builtin_isfinite(uy) && __builtin_isfinite(uz))
+                  ? 0.0
+                  : 1.0;
     }
     sm[threadIdx.x] = am;
     sk[threadIdx.x] = ak;
+    sb[threadIdx.x] = ab;
     __syncthreads();
     for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
             sm[threadIdx.x] += sm[threadIdx.x + s];
             sk[threadIdx.x] += sk[threadIdx.x + s];
+            sb[threadIdx.x] += sb[threadIdx.x + s];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = sm[0];
-        partial[2 * blockIdx.x + 1] = sk[0];
+        partial[3 * blockIdx.x] = sm[0];
+        partial[3 * blockIdx.x + 1] = sk[0];
+        partial[3 * blockIdx.x + 2] = sb[0];
     }
 }
 
-__global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double* partial, int n, double* out2) {
-    __shared__ double sm[RED_THREADS], sk[RED_THREADS];
-    double am = 0.0, ak = 0.0;
-    for (int j = threadIdx.x; j < n; j += RED_THREADS) {
-        am += partial[2 * j];
-        ak += partial[2 * j + 1];
-    }
-    sm[threadIdx.x] = am;
-    sk[threadIdx.x] = ak;
+// Fold n partial triples in a fixed order into out3; with n_nodes > 0 the
+// first two (the sums) are divided by it.
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double* partial, int n, double* out3,
+                                                              double n_nodes) {
+    __shared__ double s3[3][RED_THREADS];
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int j = threadIdx.x; j < n; j += RED_THREADS)
+        for (int k = 0; k < 3; k++) a[k] += partial[3 * j + k];
+    for (int k = 0; k < 3; k++) s3[k][threadIdx.x] = a[k];
     __syncthreads();
     for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            sm[threadIdx.x] += sm[threadIdx.x + s];
-            sk[threadIdx.x] += sk[threadIdx.x + s];
-        }
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 3; k++) s3[k][threadIdx.x] += s3[k][threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        out2[0] = sm[0];
-        out2[1] = sk[0];
-    }
+    if (threadIdx.x < 3) out3[threadIdx.x] = (threadIdx.x < 2 && n_nodes > 0.0) ? s3[threadIdx.x][0] / n_nodes
+                                                                                 : s3[threadIdx.x][0];
 }
 
 hipError_t launch_reduce(int storage, const void* mom, size_t ntiles, int tile_nodes, float rho0, float rs0, float rs1,
-                         double* partial, int nblocks, double* out2, hipStream_t s) {
+                         double* partial, int nblocks, double* out3, hipStream_t s, double n_nodes) {
     if (storage == 0)
         hipLaunchKernelGGL(k_reduce_partial<0>, dim3(nblocks), dim3(RED_THREADS), 0, s, mom, ntiles, tile_nodes, rho0,
                            rs0, rs1, partial);
@@ -80,7 +86,67 @@ hipError_t launch_reduce(int storage, const void* mom, size_t ntiles, int tile_n
         return hipErrorInvalidValue;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, s, partial, nblocks, out2);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, s, partial, nblocks, out3, n_nodes);
+    return hipGetLastError();
+}
+
+// Convergence residual: per block, sum over nodes of |u - u_prev|^2 and of
+// |u|^2 in FP64 (u from the stored moments, u_prev a saved FP32 copy,
+// 3 planes of N values in block layout); u_prev is then overwritten with u.
+// partial[3 b + 0..1] (the third slot is zero, so k_reduce_final folds it).
+template <int S>
+__global__ __launch_bounds__(RED_THREADS) void k_residual_partial(const void* mom_, float* uprev, size_t ntiles, int T,
+                                                                 float rs1, double* partial) {
+    using C = Codec<S>;
+    const typename C::T* mom = (const typename C::T*)mom_;
+    __shared__ double sd[RED_THREADS], su[RED_THREADS];
+    double ad = 0.0, au = 0.0;
+    const size_t N = ntiles * (size_t)T;
+    for (size_t j = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; j < N; j += (size_t)gridDim.x * RED_THREADS) {
+        const size_t tile = j / T, t = j % T;
+        const size_t b = tile * 10 * T + t;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float u = C::ld(mom, b + (size_t)(1 + a) * T, rs1) * (1.0f / 3.0f);
+            const double d = (double)u - (double)uprev[(size_t)a * N + j];
+            ad += d * d;
+            au += (double)u * (double)u;
+            uprev[(size_t)a * N + j] = u;
+        }
+    }
+    sd[threadIdx.x] = ad;
+    su[threadIdx.x] = au;
+    __syncthreads();
+    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sd[threadIdx.x] += sd[threadIdx.x + s];
+            su[threadIdx.x] += su[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[3 * blockIdx.x] = sd[0];
+        partial[3 * blockIdx.x + 1] = su[0];
+        partial[3 * blockIdx.x + 2] = 0.0;
+    }
+}
+
+hipError_t launch_residual(int storage, const void* mom, float* uprev, size_t ntiles, int tile_nodes, float rs1,
+                           double* partial, int nblocks, double* out3, hipStream_t s) {
+    if (storage == 0)
+        hipLaunchKernelGGL(k_residual_partial<0>, dim3(nblocks), dim3(RED_THREADS), 0, s, mom, uprev, ntiles,
+                           tile_nodes, rs1, partial);
+    else if (storage == 1)
+        hipLaunchKernelGGL(k_residual_partial<1>, dim3(nblocks), dim3(RED_THREADS), 0, s, mom, uprev, ntiles,
+                           tile_nodes, rs1, partial);
+    else if (storage == 2)
+        hipLaunchKernelGGL(k_residual_partial<2>, dim3(nblocks), dim3(RED_THREADS), 0, s, mom, uprev, ntiles,
+                           tile_nodes, rs1, partial);
+    else
+        return hipErrorInvalidValue;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, s, partial, nblocks, out3, 0.0);
     return hipGetLastError();
 }
 

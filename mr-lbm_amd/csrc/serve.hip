@@ -16,6 +16,13 @@
 // its FP64 mass (and optionally its moments) are read during the next round
 // and its memory is released at the boundary after that. Latency is arrival
 // to results-on-host.
+// Failures are per request: a request that cannot be served (extents not a
+// multiple of the tile, more than 8 spheres, no steps, larger than the whole
+// budget, an admission that fails) or whose lattice turns non-finite (an
+// FP64 reduction of every member after every round counts non-finite
+// nodes) is marked failed with its status code and its memory released,
+// while the others continue. Only device-level errors (a launch, a round,
+// batch membership) end the loop for all.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -58,12 +65,25 @@ struct Active {
     double nodes;
 };
 
+// Request that cannot be served at all (checked before the loop runs).
+int request_error(const mrlbm_request& r, const mrlbm_serve_params* sp, int bx, int by, int bz) {
+    if (r.nx <= 0 || r.ny <= 0 || r.nz <= 0 || r.nx % bx || r.ny % by || r.nz % bz) return MRLBM_EINVAL;
+    if (r.nspheres < 0 || r.nspheres > 8) return MRLBM_EINVAL;
+    if (r.steps <= 0 || !(r.nu > 0.0) || !std::isfinite(r.nu)) return MRLBM_EINVAL;
+    if (state_bytes(r, sp->q, sp->storage, bx, by, bz) > sp->hbm_budget_bytes) return MRLBM_ENOMEM;
+    return MRLBM_OK;
+}
+
 // The serving loop, shared by the GPU and a simulated device. Dev supplies:
 //   double now();                           seconds since the loop started
 //   int admit(mrlbm_request&, Active&);     allocate, classify, initialise
 //   int add(Active&), remove(Active&);      batch membership
-//   int launch(int q, bool graph);          start a round of q steps
-//   int wait_round(double* busy_s);         end of the round in flight
+//   int launch(int q, bool graph, const std::vector<Active>&)
+//                                           start a round of q steps, then a
+//                                           health check of every member
+//   int wait_round(double* busy_s, std::vector<char>& bad)
+//                                           end of the round in flight; bad[i]
+//                                           = member i turned non-finite
 //   int finish(Active&, mrlbm_request&);    read results (mass, moments)
 //   void release(Active&);                  free its memory
 //   void idle(double seconds);              nothing to run until then
@@ -74,25 +94,30 @@ int serve_loop(Dev& dev, const mrlbm_serve_params* sp, mrlbm_request* reqs, int 
     const int graph = sp->graph_steps > 0 && quantum % sp->graph_steps == 0 ? sp->graph_steps : 0;
     const int max_batch = sp->max_batch > 0 ? std::min(sp->max_batch, 4095) : 4095;
     std::memset(st, 0, sizeof *st);
-    for (int i = 0; i < nreq; i++) {
-        if (reqs[i].nx <= 0 || reqs[i].ny <= 0 || reqs[i].nz <= 0 || reqs[i].nx % bx || reqs[i].ny % by ||
-            reqs[i].nz % bz)
-            return fail(MRLBM_EINVAL, "request %d extents are not positive multiples of the tile", i);
-        if (reqs[i].nspheres < 0 || reqs[i].nspheres > 8)
-            return fail(MRLBM_EINVAL, "request %d has %d spheres (0..8)", i, reqs[i].nspheres);
-        if (reqs[i].steps <= 0) return fail(MRLBM_EINVAL, "request %d has no steps", i);
-        if (state_bytes(reqs[i], sp->q, sp->storage, bx, by, bz) > sp->hbm_budget_bytes)
-            return fail(MRLBM_ENOMEM, "request %d alone exceeds the HBM budget", i);
+    for (int i = 0; i < nreq; i++)
         if (i && reqs[i].arrival_s < reqs[i - 1].arrival_s)
             return fail(MRLBM_EINVAL, "requests must be ordered by arrival time");
-        reqs[i].status = 1;  // pending
+    for (int i = 0; i < nreq; i++) {
+        mrlbm_request& r = reqs[i];
+        r.start_s = r.finish_s = r.latency_s = 0.0;
+        r.status = 1;  // pending
     }
     // ready: admitted and initialised, joins at the next round boundary;
     // active: batch members; leaving: out of the batch, results being read
     std::vector<Active> ready, active, leaving;
     double used = 0, peak = 0, busy = 0, lu = 0;
     int next = 0, done = 0, round_q = 0, rc = 0;
+    int64_t failed = 0;
     bool in_flight = false;
+    std::vector<char> bad;
+    // a request leaves the loop failed with status `code` (negative)
+    auto fail_request = [&](mrlbm_request& r, int code) {
+        r.status = code;
+        r.finish_s = dev.now();
+        r.latency_s = 0.0;
+        done++;
+        failed++;
+    };
 
     auto release_all = [&]() {
         for (auto* v : {&ready, &active, &leaving})
@@ -109,11 +134,21 @@ int serve_loop(Dev& dev, const mrlbm_serve_params* sp, mrlbm_request* reqs, int 
         while (next < nreq && (!sp->realtime || reqs[next].arrival_s <= tnow) &&
                (int)(active.size() + ready.size()) < max_batch) {
             mrlbm_request& r = reqs[next];
+            const int bad_req = request_error(r, sp, bx, by, bz);
+            if (bad_req) {
+                next++;
+                fail_request(r, bad_req);
+                continue;
+            }
             const double bytes = state_bytes(r, sp->q, sp->storage, bx, by, bz);
             if (used + bytes > sp->hbm_budget_bytes) break;
             Active a{next, nullptr, r.steps, bytes, (double)r.nx * r.ny * r.nz};
             const int e = dev.admit(r, a);
-            if (e) return e;
+            if (e) {
+                next++;
+                fail_request(r, e);
+                continue;
+            }
             r.start_s = dev.now();
             r.status = 2;  // running
             ready.push_back(a);
@@ -127,7 +162,10 @@ int serve_loop(Dev& dev, const mrlbm_serve_params* sp, mrlbm_request* reqs, int 
         for (auto& a : leaving) {
             mrlbm_request& r = reqs[a.idx];
             const int e = dev.finish(a, r);
-            if (e) return e;
+            if (e) {
+                fail_request(r, e);
+                continue;
+            }
             r.finish_s = dev.now();
             r.latency_s = r.finish_s - (sp->realtime ? r.arrival_s : 0.0);
             r.status = 0;
@@ -141,13 +179,22 @@ int serve_loop(Dev& dev, const mrlbm_serve_params* sp, mrlbm_request* reqs, int 
         if ((rc = finish())) return fail_all(rc);
         if (in_flight) {
             double b = 0;
-            if ((rc = dev.wait_round(&b))) return fail_all(rc);
+            if ((rc = dev.wait_round(&b, bad))) return fail_all(rc);
             busy += b;
             st->rounds++;
             in_flight = false;
             for (auto& a : active) {
                 a.remaining -= round_q;
                 lu += (double)round_q * a.nodes;
+            }
+            // members that turned non-finite leave failed; the others go on
+            for (size_t i = active.size(); i-- > 0;) {
+                if (!bad[i]) continue;
+                if ((rc = dev.remove(active[i]))) return fail_all(rc);
+                dev.release(active[i]);
+                used -= active[i].bytes;
+                fail_request(reqs[active[i].idx], MRLBM_ENUMERIC);
+                active.erase(active.begin() + i);
             }
             // requests that arrived during the round join the next one
             // (the first pass ran as the round started)
@@ -185,14 +232,15 @@ int serve_loop(Dev& dev, const mrlbm_serve_params* sp, mrlbm_request* reqs, int 
         // next round: at most `quantum` steps, never past a member's budget
         int q = quantum;
         for (auto& a : active) q = std::min(q, a.remaining);
-        if ((rc = dev.launch(q, graph && q == quantum))) return fail_all(rc);
+        if ((rc = dev.launch(q, graph && q == quantum, active))) return fail_all(rc);
         round_q = q;
         in_flight = true;
     }
     release_all();
     const double wall = dev.now();
     std::vector<double> lat;
-    for (int i = 0; i < nreq; i++) lat.push_back(reqs[i].latency_s);
+    for (int i = 0; i < nreq; i++)
+        if (reqs[i].status == 0) lat.push_back(reqs[i].latency_s);
     std::sort(lat.begin(), lat.end());
     auto pct = [&](double p) {
         if (lat.empty()) return 0.0;
@@ -210,6 +258,7 @@ int serve_loop(Dev& dev, const mrlbm_serve_params* sp, mrlbm_request* reqs, int 
     st->p99_s = pct(0.99);
     st->max_s = lat.empty() ? 0 : lat.back();
     st->peak_bytes = peak;
+    st->failed = failed;
     return MRLBM_OK;
 }
 
@@ -219,6 +268,10 @@ struct GpuDevice {
     int bx, by, bz;
     mrlbm_batch* batch = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    static constexpr int kMaxMembers = 4096;
+    double* d_health = nullptr;  // 3 doubles per member: FP64 mass, KE, non-finite nodes
+    double* h_health = nullptr;  // pinned copy
+    int n_checked = 0;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
@@ -253,22 +306,40 @@ struct GpuDevice {
     }
     int add(Active& a) { return mrlbm_batch_add(batch, a.ctx); }
     int remove(Active& a) { return mrlbm_batch_remove(batch, a.ctx); }
-    int launch(int q, bool use_graph) {
-        if (hipEventRecord(ev0, batch_stream(batch)) != hipSuccess) return fail(MRLBM_EHIP, "event record");
+    int launch(int q, bool use_graph, const std::vector<Active>& active) {
+        const hipStream_t s = batch_stream(batch);
+        if (hipEventRecord(ev0, s) != hipSuccess) return fail(MRLBM_EHIP, "event record");
         const int e = mrlbm_batch_step(batch, q, use_graph ? sp->graph_steps : 0);
         if (e) return e;
-        if (hipEventRecord(ev1, batch_stream(batch)) != hipSuccess) return fail(MRLBM_EHIP, "event record");
+        // health of every member after the round, on the batch stream
+        n_checked = (int)active.size();
+        for (int i = 0; i < n_checked; i++) {
+            const mrlbm_ctx* c = active[i].ctx;
+            const hipError_t he = launch_reduce(c->p.storage, c->d_mom, c->ntiles, c->T, (float)c->p.rho0,
+                                                c->mrscale[0], c->mrscale[1], c->d_partial, c->red_blocks,
+                                                d_health + 3 * i, s);
+            if (he != hipSuccess) return fail(MRLBM_EHIP, "health reduction: %s", hipGetErrorString(he));
+        }
+        if (n_checked &&
+            hipMemcpyAsync(h_health, d_health, sizeof(double) * 3 * n_checked, hipMemcpyDeviceToHost, s) != hipSuccess)
+            return fail(MRLBM_EHIP, "health read-back");
+        if (hipEventRecord(ev1, s) != hipSuccess) return fail(MRLBM_EHIP, "event record");
         return MRLBM_OK;
     }
-    int wait_round(double* busy_s) {
+    int wait_round(double* busy_s, std::vector<char>& bad) {
         if (hipEventSynchronize(ev1) != hipSuccess) return fail(MRLBM_EHIP, "round failed");
         float ms = 0;
         (void)hipEventElapsedTime(&ms, ev0, ev1);
         *busy_s = ms * 1e-3;
+        bad.assign(n_checked, 0);
+        for (int i = 0; i < n_checked; i++)
+            bad[i] = h_health[3 * i + 2] != 0.0 || !std::isfinite(h_health[3 * i]) || !std::isfinite(h_health[3 * i + 1]);
         return MRLBM_OK;
     }
     int finish(Active& a, mrlbm_request& r) {
-        int e = mrlbm_reduce(a.ctx, &r.mass1, &r.ke1);
+        int64_t nonfinite = 0;
+        int e = mrlbm_reduce_checked(a.ctx, &r.mass1, &r.ke1, &nonfinite);
+        if (!e && nonfinite) e = fail(MRLBM_ENUMERIC, "request %d: %lld non-finite nodes", a.idx, (long long)nonfinite);
         if (!e && r.out_moments) e = mrlbm_read_moments(a.ctx, r.out_moments);
         return e;
     }
@@ -284,8 +355,10 @@ struct GpuDevice {
 // admit_s seconds of host time; results are not computed.
 struct SimDevice {
     double rate_lups, admit_s;
+    const mrlbm_request* reqs;
     double clock = 0, round_end = 0, round_len = 0;
     double batch_nodes = 0;
+    std::vector<char> round_bad;
     double now() const { return clock; }
     int admit(mrlbm_request& r, Active&) {
         clock += admit_s;
@@ -300,14 +373,19 @@ struct SimDevice {
         batch_nodes -= a.nodes;
         return MRLBM_OK;
     }
-    int launch(int q, bool) {
+    // a lattice driven by a non-finite force is non-finite after its first
+    // round, as on the GPU
+    int launch(int q, bool, const std::vector<Active>& active) {
         round_len = batch_nodes * q / rate_lups;
         round_end = clock + round_len;
+        round_bad.assign(active.size(), 0);
+        for (size_t i = 0; i < active.size(); i++) round_bad[i] = !std::isfinite(reqs[active[i].idx].gx);
         return MRLBM_OK;
     }
-    int wait_round(double* busy_s) {
+    int wait_round(double* busy_s, std::vector<char>& bad) {
         clock = std::max(clock, round_end);
         *busy_s = round_len;
+        bad = round_bad;
         return MRLBM_OK;
     }
     int finish(Active&, mrlbm_request&) { return MRLBM_OK; }
@@ -359,15 +437,22 @@ int mrlbm_serve_run(const mrlbm_serve_params* sp, mrlbm_request* reqs, int nreq,
     dev.bz = sp->bz ? sp->bz : 8;
     int rc = mrlbm_batch_create(sp->device, sp->q, dev.bx, dev.by, dev.bz, sp->storage, &dev.batch);
     if (rc) return rc;
-    if (hipEventCreate(&dev.ev0) != hipSuccess || hipEventCreate(&dev.ev1) != hipSuccess) {
+    auto cleanup = [&]() {
         mrlbm_batch_destroy(dev.batch);
-        return fail(MRLBM_EHIP, "cannot create events");
+        if (dev.ev0) (void)hipEventDestroy(dev.ev0);
+        if (dev.ev1) (void)hipEventDestroy(dev.ev1);
+        if (dev.d_health) (void)hipFree(dev.d_health);
+        if (dev.h_health) (void)hipHostFree(dev.h_health);
+    };
+    if (hipEventCreate(&dev.ev0) != hipSuccess || hipEventCreate(&dev.ev1) != hipSuccess ||
+        hipMalloc((void**)&dev.d_health, sizeof(double) * 3 * GpuDevice::kMaxMembers) != hipSuccess ||
+        hipHostMalloc((void**)&dev.h_health, sizeof(double) * 3 * GpuDevice::kMaxMembers) != hipSuccess) {
+        cleanup();
+        return fail(MRLBM_EHIP, "cannot create the serving loop's events and health buffers");
     }
     dev.t0 = std::chrono::steady_clock::now();
     rc = serve_loop(dev, sp, reqs, nreq, st);
-    mrlbm_batch_destroy(dev.batch);
-    (void)hipEventDestroy(dev.ev0);
-    (void)hipEventDestroy(dev.ev1);
+    cleanup();
     return rc;
 }
 
@@ -375,7 +460,7 @@ int mrlbm_serve_simulate(const mrlbm_serve_params* sp, mrlbm_request* reqs, int 
                          double admit_s, mrlbm_serve_stats* st) {
     if (!sp || (!reqs && nreq) || !st || nreq < 0 || !(rate_mlups > 0) || !(admit_s >= 0))
         return fail(MRLBM_EINVAL, "bad argument");
-    SimDevice dev{rate_mlups * 1e6, admit_s};
+    SimDevice dev{rate_mlups * 1e6, admit_s, reqs};
     return serve_loop(dev, sp, reqs, nreq, st);
 }
 

@@ -58,21 +58,36 @@ __device__ __forceinline__ void decode_node(const StepArgs& a, const typename Fe
 // the ones its neighbours would reconstruct from the stored moments: results
 // do not depend on the tile shape. FP32 storage is exact.
 //
-// Stochastic rounding: v * scale (a power of two, exact) gets uniform random
-// bits added below the kept mantissa bits, then those bits are cleared,
-// giving t (FP16: 10 kept mantissa bits; BF16: 7). BF16 has FP32's exponent
-// range, so t is the stored value exactly: the stored bits are t's upper 16
-// bits and the moment is t / scale. FP16 packs pairs of t by a round-toward-
-// zero conversion (exact for t in the FP16 normal range) and the moments are
-// decoded back from the packed halves, so values below the normal range,
-// which the conversion truncates further, are also used as stored. 8 random
-// bits per value come from three words: one counter-based hash (sr_bits)
-// and two xorshift successors.
+// Stochastic rounding by gfx950's conversion instructions
+// (v_cvt_sr_f16_f32, v_cvt_sr_bf16_f32): the random operand's low bits
+// stand for the dropped mantissa bits (13 for binary16, 16 for bfloat16)
+// and decide the rounding direction, so the result is v rounded up with
+// probability equal to its dropped fraction. 8 random bits per value,
+// placed at the top of that field (the bits below come from the
+// neighbouring value's byte), are drawn from three words: one
+// counter-based hash of (node, step, seed) (sr_bits) and two xorshift
+// successors, so the same node, step and seed round alike in any launch
+// shape or batch. binary16 stores v times a power-of-two scale (exact) and
+// the moments are decoded back from the packed halves; bfloat16 has FP32's
+// exponent range and is stored unscaled.
 __device__ __forceinline__ uint32_t xs32(uint32_t x) {
     x ^= x << 13;
     x ^= x >> 17;
     x ^= x << 5;
     return x;
+}
+
+// Random operand of byte B of word w for a conversion dropping DROP bits:
+// that byte moved to bits [DROP - 8, DROP).
+template <int DROP, int B>
+__device__ __forceinline__ uint32_t sr_operand(uint32_t w) {
+    constexpr int sh = 8 * B - (DROP - 8);
+    if constexpr (sh > 0)
+        return w >> sh;
+    else if constexpr (sh < 0)
+        return w << (-sh);
+    else
+        return w;
 }
 
 template <int S>
@@ -90,42 +105,40 @@ __device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t st
         q[8] = n.myz;
         q[9] = n.mzz;
     } else {
+        typedef _Float16 __attribute__((ext_vector_type(2))) half2_t;
+        typedef __bf16 __attribute__((ext_vector_type(2))) bf162_t;
+        constexpr int DROP = S == 1 ? 13 : 16;
         const uint32_t w0 = sr_bits(node_id, step, a.seed);
         const uint32_t w1 = xs32(w0), w2 = xs32(w1);
-        constexpr uint32_t SH = S == 1 ? 5u : 8u;         // random byte -> dropped-bit position
-        constexpr uint32_t HALF = S == 1 ? 16u : 128u;    // centre of the byte's interval
-        constexpr uint32_t KEEP = S == 1 ? ~0x1FFFu : ~0xFFFFu;
         float* v[10] = {&n.drho, &n.ux, &n.uy, &n.uz, &n.mxx, &n.mxy, &n.mxz, &n.myy, &n.myz, &n.mzz};
-        const uint32_t w[3] = {w0, w1, w2};
-        float t[10];
-#pragma unroll
-        for (int k = 0; k < 10; k++) {
-            const uint32_t r = (((w[k / 4] >> (8 * (k % 4))) & 0xFFu) << SH);
-            // binary16: the power-of-two scale 2^e enters as e added to the
-            // exponent field (uniform), which equals the bits of v * 2^e
-            // for every normal v; zero and FP32-subnormal v become values
-            // below 2^-100 that the conversion below flushes to the same
-            // signed zero as before. bfloat16 is stored unscaled.
-            const uint32_t e = (__builtin_bit_cast(uint32_t, store_scale(S, k)) - 0x3F800000u) + HALF;
-            t[k] = __uint_as_float((__float_as_uint(*v[k]) + r + e) & KEEP);
-        }
-        if constexpr (S == 1) {
-#pragma unroll
-            for (int k = 0; k < 10; k += 2) {
-                const auto p = __builtin_amdgcn_cvt_pkrtz(t[k], t[k + 1]);
-                const uint32_t b = __builtin_bit_cast(uint32_t, p);
-                q[k] = (uint16_t)(b & 0xFFFFu);
-                q[k + 1] = (uint16_t)(b >> 16);
-                *v[k] = f16_scaled<false>(b, 1.0f / store_scale(S, k));
-                *v[k + 1] = f16_scaled<true>(b, 1.0f / store_scale(S, k + 1));
+        static_for<5>([&](auto PP) {
+            constexpr int k = 2 * decltype(PP)::value;
+            auto rnd = [&](auto KK) {
+                constexpr int kk = decltype(KK)::value;
+                const uint32_t w = kk < 4 ? w0 : (kk < 8 ? w1 : w2);
+                return sr_operand<DROP, kk % 4>(w);
+            };
+            uint32_t b;
+            if constexpr (S == 1) {
+                constexpr float s0 = store_scale(1, k), s1 = store_scale(1, k + 1);
+                half2_t h = {};
+                h = __builtin_amdgcn_cvt_sr_f16_f32(h, *v[k] * s0, rnd(std::integral_constant<int, k>()), false);
+                h = __builtin_amdgcn_cvt_sr_f16_f32(h, *v[k + 1] * s1, rnd(std::integral_constant<int, k + 1>()),
+                                                    true);
+                b = __builtin_bit_cast(uint32_t, h);
+                *v[k] = f16_scaled<false>(b, 1.0f / s0);
+                *v[k + 1] = f16_scaled<true>(b, 1.0f / s1);
+            } else {
+                bf162_t h = {};
+                h = __builtin_amdgcn_cvt_sr_bf16_f32(h, *v[k], rnd(std::integral_constant<int, k>()), false);
+                h = __builtin_amdgcn_cvt_sr_bf16_f32(h, *v[k + 1], rnd(std::integral_constant<int, k + 1>()), true);
+                b = __builtin_bit_cast(uint32_t, h);
+                *v[k] = __uint_as_float(b << 16);
+                *v[k + 1] = __uint_as_float(b & 0xFFFF0000u);
             }
-        } else {
-#pragma unroll
-            for (int k = 0; k < 10; k++) {
-                q[k] = (uint16_t)(__float_as_uint(t[k]) >> 16);
-                *v[k] = t[k];
-            }
-        }
+            q[k] = (uint16_t)(b & 0xFFFFu);
+            q[k + 1] = (uint16_t)(b >> 16);
+        });
         n.rho = a.rho0 + n.drho;
     }
 }

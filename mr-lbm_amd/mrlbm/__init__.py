@@ -137,6 +137,18 @@ class Lattice:
         _lib.check(_lib.load().mrlbm_reduce(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def reduce_checked(self):
+        """(mean mass, mean KE, number of non-finite nodes), FP64."""
+        a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        _lib.check(_lib.load().mrlbm_reduce_checked(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)))
+        return a.value, b.value, n.value
+
+    def velocity_residual(self):
+        """||u - u_prev|| / ||u|| since the previous call (-1 on the first)."""
+        r = ctypes.c_double()
+        _lib.check(_lib.load().mrlbm_velocity_residual(self._h, ctypes.byref(r)))
+        return r.value
+
     # -- diagnostics ----------------------------------------------------
     def sample(self, points):
         """Stored-convention moments (n, 10) at integer nodes points (n, 3)."""

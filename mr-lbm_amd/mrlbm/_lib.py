@@ -60,7 +60,7 @@ class ServeStats(ctypes.Structure):
         ("wall_s", ctypes.c_double), ("busy_s", ctypes.c_double), ("lattice_updates", ctypes.c_double),
         ("mlups", ctypes.c_double), ("mlups_busy", ctypes.c_double), ("p50_s", ctypes.c_double),
         ("p99_s", ctypes.c_double), ("max_s", ctypes.c_double), ("peak_bytes", ctypes.c_double),
-        ("rounds", ctypes.c_int64),
+        ("rounds", ctypes.c_int64), ("failed", ctypes.c_int64),
     ]
 
 
@@ -97,6 +97,8 @@ _SIGS = {
     "mrlbm_read_moments": [_vp, _dp],
     "mrlbm_write_moments": [_vp, _dp],
     "mrlbm_reduce": [_vp, _dp, _dp],
+    "mrlbm_reduce_checked": [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64)],
+    "mrlbm_velocity_residual": [_vp, _dp],
     "mrlbm_checkpoint_save": [_vp, ctypes.c_char_p],
     "mrlbm_checkpoint_load": [_vp, ctypes.c_char_p],
     "mrlbm_save_vtk": [_vp, ctypes.c_char_p],

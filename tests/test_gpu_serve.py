@@ -65,7 +65,20 @@ def test_realtime_stream_stats():
     assert st["rounds"] < sum(-(-r.steps // 100) for r in reqs)
 
 
-def test_budget_rejects_oversized_request():
-    reqs = small_stream(1, 10.0, 1)
-    with pytest.raises(mrlbm.MrlbmError):
-        serve.run(reqs, budget_gb=1e-4, realtime=False)
+def test_failed_requests_leave_the_others_unchanged():
+    """One request larger than the budget (ENOMEM), one with extents off the
+    tile (EINVAL) and one driven by a NaN force, whose lattice turns
+    non-finite in its first round (ENUMERIC, found by the per-round FP64
+    check and released): the others finish bit-identical to lone runs."""
+    reqs = small_stream(6, 1000.0, 5)
+    reqs[1].nx, reqs[1].ny, reqs[1].nz = 256, 128, 128  # 4.2 M nodes, ~214 MB: over the 0.1 GB budget
+    reqs[2].nx = 70
+    reqs[4].gx = float("nan")
+    bufs = [serve.attach_output(r) if i not in (1, 2) else None for i, r in enumerate(reqs)]
+    st = serve.run(reqs, storage=mrlbm.STORE_FP16, budget_gb=0.1, quantum=50, graph_steps=25, realtime=False)
+    say("stats", st, [r.status for r in reqs])
+    assert [r.status for r in reqs] == [0, -3, -1, 0, -6, 0]
+    assert st["failed"] == 3
+    for i in (0, 3, 5):
+        assert np.array_equal(bufs[i].reshape(10, -1), alone(reqs[i], mrlbm.STORE_FP16)), i
+        assert np.isfinite(reqs[i].mass1)

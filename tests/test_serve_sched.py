@@ -110,7 +110,48 @@ def test_batch_limit_respected():
     assert peak <= 3
 
 
-def test_rejects_request_over_budget():
+def test_request_over_budget_fails_alone():
+    """A request larger than the whole budget fails (MRLBM_ENOMEM) at its
+    arrival; the one that fits is served as if it were alone."""
     reqs = two_requests(10, 10)
+    reqs[0].nx = 256  # 256 x 32 x 32 does not fit 0.8 MB; 64 x 32 x 32 does
+    st = serve.simulate(reqs, 1000.0, budget_gb=state_bytes(reqs[1]) * 1.01e-9)
+    assert reqs[0].status == -3 and reqs[1].status == 0
+    assert st["failed"] == 1
+    assert st["lattice_updates"] == 64 * 32 * 32 * 10
+    assert st["p50_s"] == pytest.approx(reqs[1].latency_s)
+
+
+def test_failures_are_per_request():
+    """Among 40 requests: one whose extents are not a multiple of the tile
+    (EINVAL), one with nine spheres (EINVAL), one with nu = 0 (EINVAL) and
+    one driven by a non-finite force, which turns non-finite in its first
+    round (ENUMERIC) and leaves the batch then. Every other request runs
+    exactly its step budget; the work done is theirs plus one round of the
+    non-finite one; percentiles count completed requests only."""
+    reqs = serve.duct_requests(40, rate=20.0, seed=13)
+    reqs[3].nx = 70
+    reqs[7].nspheres = 9
+    reqs[11].nu = 0.0
+    reqs[17].gx = float("nan")
+    st = serve.simulate(reqs, 40000.0, admit_s=0.001, quantum=100, max_batch=6)
+    bad = {3: -1, 7: -1, 11: -1, 17: -6}
+    for i, r in enumerate(reqs):
+        assert r.status == bad.get(i, 0), i
+    assert st["failed"] == 4
+    ok = [r for i, r in enumerate(reqs) if i not in bad]
+    first_round = min(100, reqs[17].steps)
+    total = sum(r.nx * r.ny * r.nz * r.steps for r in ok) + reqs[17].nx * reqs[17].ny * reqs[17].nz * first_round
+    assert st["lattice_updates"] == total
+    lat = sorted(r.latency_s for r in ok)
+    assert st["p50_s"] == pytest.approx(np.percentile(lat, 50), rel=1e-12)
+    assert st["max_s"] == pytest.approx(lat[-1])
+    # the non-finite request left after one round: its finish follows its start by one round
+    assert reqs[17].finish_s > reqs[17].start_s
+
+
+def test_unordered_stream_is_rejected():
+    reqs = two_requests(10, 10)
+    reqs[0].arrival_s, reqs[1].arrival_s = 1.0, 0.5
     with pytest.raises(_lib.MrlbmError):
-        serve.simulate(reqs, 1000.0, budget_gb=1e-3)
+        serve.simulate(reqs, 1000.0)
