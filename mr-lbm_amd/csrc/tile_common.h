@@ -59,35 +59,30 @@ __device__ __forceinline__ void decode_node(const StepArgs& a, const typename Fe
 // do not depend on the tile shape. FP32 storage is exact.
 //
 // Stochastic rounding by gfx950's conversion instructions
-// (v_cvt_sr_f16_f32, v_cvt_sr_bf16_f32): the random operand's low bits
-// stand for the dropped mantissa bits (13 for binary16, 16 for bfloat16)
-// and decide the rounding direction, so the result is v rounded up with
-// probability equal to its dropped fraction. 8 random bits per value,
-// placed at the top of that field (the bits below come from the
-// neighbouring value's byte), are drawn from three words: one
-// counter-based hash of (node, step, seed) (sr_bits) and two xorshift
-// successors, so the same node, step and seed round alike in any launch
-// shape or batch. binary16 stores v times a power-of-two scale (exact) and
-// the moments are decoded back from the packed halves; bfloat16 has FP32's
+// (v_cvt_sr_f16_f32, v_cvt_sr_bf16_f32). They round up when the dropped
+// fraction plus the random operand, read as a fraction of 2^32, reaches one
+// (measured on MI355X: for a value halfway between two neighbours only
+// operands with bit 31 set round up), so the operand's top bits decide and
+// each value needs its own top bits. Ten 32-bit operands with ten distinct
+// top bytes come from three words: a counter-based hash of (node, step,
+// seed) (sr_bits), two multiplicative successors, and funnel shifts of
+// word pairs; the same node, step and seed round alike in any launch shape
+// or batch. binary16 stores v times a power-of-two scale (exact) and the
+// moments are decoded back from the packed halves; bfloat16 has FP32's
 // exponent range and is stored unscaled.
-__device__ __forceinline__ uint32_t xs32(uint32_t x) {
-    x ^= x << 13;
-    x ^= x >> 17;
-    x ^= x << 5;
-    return x;
-}
-
-// Random operand of byte B of word w for a conversion dropping DROP bits:
-// that byte moved to bits [DROP - 8, DROP).
-template <int DROP, int B>
-__device__ __forceinline__ uint32_t sr_operand(uint32_t w) {
-    constexpr int sh = 8 * B - (DROP - 8);
-    if constexpr (sh > 0)
-        return w >> sh;
-    else if constexpr (sh < 0)
-        return w << (-sh);
-    else
-        return w;
+template <int K>
+__device__ __forceinline__ uint32_t sr_operand(uint32_t w0, uint32_t w1, uint32_t w2) {
+    // top bytes: w0.b3, w0.b0, w0.b1, w0.b2, w1.b3, w1.b0, w2.b0, w2.b1, w2.b2, w2.b3
+    if constexpr (K == 0) return w0;
+    else if constexpr (K == 1) return __builtin_amdgcn_alignbit(w0, w1, 8);
+    else if constexpr (K == 2) return __builtin_amdgcn_alignbit(w0, w1, 16);
+    else if constexpr (K == 3) return __builtin_amdgcn_alignbit(w0, w1, 24);
+    else if constexpr (K == 4) return w1;
+    else if constexpr (K == 5) return __builtin_amdgcn_alignbit(w1, w2, 8);
+    else if constexpr (K == 6) return __builtin_amdgcn_alignbit(w2, w0, 8);
+    else if constexpr (K == 7) return __builtin_amdgcn_alignbit(w2, w0, 16);
+    else if constexpr (K == 8) return __builtin_amdgcn_alignbit(w2, w0, 24);
+    else return w2;
 }
 
 template <int S>
@@ -107,31 +102,25 @@ __device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t st
     } else {
         typedef _Float16 __attribute__((ext_vector_type(2))) half2_t;
         typedef __bf16 __attribute__((ext_vector_type(2))) bf162_t;
-        constexpr int DROP = S == 1 ? 13 : 16;
         const uint32_t w0 = sr_bits(node_id, step, a.seed);
-        const uint32_t w1 = xs32(w0), w2 = xs32(w1);
+        const uint32_t w1 = w0 * 0x9E3779B1u, w2 = w0 * 0x85EBCA77u;
         float* v[10] = {&n.drho, &n.ux, &n.uy, &n.uz, &n.mxx, &n.mxy, &n.mxz, &n.myy, &n.myz, &n.mzz};
         static_for<5>([&](auto PP) {
             constexpr int k = 2 * decltype(PP)::value;
-            auto rnd = [&](auto KK) {
-                constexpr int kk = decltype(KK)::value;
-                const uint32_t w = kk < 4 ? w0 : (kk < 8 ? w1 : w2);
-                return sr_operand<DROP, kk % 4>(w);
-            };
+            const uint32_t r0 = sr_operand<k>(w0, w1, w2), r1 = sr_operand<k + 1>(w0, w1, w2);
             uint32_t b;
             if constexpr (S == 1) {
                 constexpr float s0 = store_scale(1, k), s1 = store_scale(1, k + 1);
                 half2_t h = {};
-                h = __builtin_amdgcn_cvt_sr_f16_f32(h, *v[k] * s0, rnd(std::integral_constant<int, k>()), false);
-                h = __builtin_amdgcn_cvt_sr_f16_f32(h, *v[k + 1] * s1, rnd(std::integral_constant<int, k + 1>()),
-                                                    true);
+                h = __builtin_amdgcn_cvt_sr_f16_f32(h, *v[k] * s0, r0, false);
+                h = __builtin_amdgcn_cvt_sr_f16_f32(h, *v[k + 1] * s1, r1, true);
                 b = __builtin_bit_cast(uint32_t, h);
                 *v[k] = f16_scaled<false>(b, 1.0f / s0);
                 *v[k + 1] = f16_scaled<true>(b, 1.0f / s1);
             } else {
                 bf162_t h = {};
-                h = __builtin_amdgcn_cvt_sr_bf16_f32(h, *v[k], rnd(std::integral_constant<int, k>()), false);
-                h = __builtin_amdgcn_cvt_sr_bf16_f32(h, *v[k + 1], rnd(std::integral_constant<int, k + 1>()), true);
+                h = __builtin_amdgcn_cvt_sr_bf16_f32(h, *v[k], r0, false);
+                h = __builtin_amdgcn_cvt_sr_bf16_f32(h, *v[k + 1], r1, true);
                 b = __builtin_bit_cast(uint32_t, h);
                 *v[k] = __uint_as_float(b << 16);
                 *v[k + 1] = __uint_as_float(b & 0xFFFF0000u);

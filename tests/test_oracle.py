@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import _oracle as O
-from fields import rel_l2
+from fields import drho_rel, rel_l2
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1])
@@ -230,6 +230,25 @@ def test_rel_l2_helper_detects_difference():
     b = a.copy()
     b[1, 0] += 1e-3
     assert rel_l2(a, b)["u"] > 1e-3
+
+
+def test_rel_l2_density_scale():
+    """The density-deviation error is relative to the deviation where it
+    exceeds the velocity, and relative to the velocity where the deviation
+    is smaller (round-off in a sum of velocity-sized populations) or zero."""
+    rng = np.random.default_rng(1)
+    b = np.zeros((10, 64))
+    b[1:4] = 3 * rng.normal(0, 0.01, (3, 64))  # stored 3u, |u| ~ 0.01
+    big = b.copy()
+    big[0] = rng.normal(0, 0.1, 64)  # |drho| >> |u|
+    a = big.copy()
+    a[0] *= 1 + 2e-5
+    assert rel_l2(a, big)["drho"] == pytest.approx(2e-5, rel=1e-6)
+    assert drho_rel(a, big) == pytest.approx(2e-5, rel=1e-6)
+    flat = b.copy()  # drho = 0 (uniform density): the velocity sets the scale
+    a = flat.copy()
+    a[0] = 3e-5 * np.linalg.norm(b[1:4]) / 3 / np.sqrt(64)
+    assert rel_l2(a, flat)["drho"] == pytest.approx(3e-5, rel=1e-6)
 
 
 def test_duct_series_limits():
