@@ -154,6 +154,22 @@ int mrlbm_reduce(mrlbm_ctx* ctx, double* mean_mass, double* mean_ke);
  * finite (a NaN or Inf anywhere also makes the sums non-finite). */
 int mrlbm_reduce_checked(mrlbm_ctx* ctx, double* mean_mass, double* mean_ke, int64_t* nonfinite_nodes);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid
+ * hipMalloc). mrlbm_reduce_checked without host synchronisation and without
+ * the context's partial-sum buffer: enqueues on the context's stream the
+ * same FP64 sums (same kernels, same block count, so the same bits) and
+ * writes d_out3[0] = mean(rho - RHO_0), d_out3[1] = mean(rho |u|^2 / 2),
+ * d_out3[2] = the number of non-finite nodes (as a double) to the caller's
+ * device buffer, which must be device memory on the context's device. The
+ * partial sums use a per-device array of 1024 FP64 triples (24 KiB) held
+ * until the process ends; the sums stride over the nodes, so every lattice
+ * fits. Calls on one device share that array: calls ordered on one stream
+ * are safe, calls that overlap on different streams of one device race.
+ * The results are ready once the context's stream reaches them
+ * (mrlbm_synchronize). */
+int mrlbm_reduce_checked_noalloc(mrlbm_ctx* ctx, double* d_out3);
+
 /* FP64 convergence residual of the velocity field: ||u - u_prev|| / ||u||
  * (L2 over all nodes), u_prev being the velocity at the previous call; the
  * first call stores the reference (12 B/node, allocated then) and returns

@@ -121,6 +121,9 @@ int mrlbm_create(const mrlbm_params* pp, mrlbm_ctx** out) {
     HIPCHK(hipGetDeviceCount(&ndev));
     if (p.device < 0 || p.device >= ndev) return fail(MRLBM_EINVAL, "device %d not present", p.device);
     HIPCHK(hipSetDevice(p.device));
+    // set-up of the device's scratch array for mrlbm_reduce_checked_noalloc
+    // (once per device: the symbol lookup is not an allocation)
+    if (int rc0 = noalloc_scratch_setup(p.device, nullptr)) return rc0;
 
     mrlbm_ctx* c = new mrlbm_ctx();
     c->p = p;

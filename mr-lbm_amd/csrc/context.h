@@ -133,4 +133,9 @@ int build_tile_list(mrlbm_ctx* c);
 // The stream a batch launches on (members' work is ordered on it).
 hipStream_t batch_stream(const mrlbm_batch* b);
 
+// Device address of the allocation-free reduction's scratch array on
+// `device` (which must be current), looked up on the first call for that
+// device and kept (reduce.hip).
+int noalloc_scratch_setup(int device, double** out);
+
 }  // namespace mrlbm

@@ -5,7 +5,10 @@
 // in FP32 with repeated block passes; here each thread accumulates in FP64,
 // one LDS tree per block, and a single block folds the partials in a fixed
 // order, so the result is deterministic for a given launch shape.
-#include "kernels.h"
+#include <mutex>
+#include <vector>
+
+#include "context.h"
 
 namespace mrlbm {
 
@@ -150,4 +153,58 @@ hipError_t launch_residual(int storage, const void* mom, float* uprev, size_t nt
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free checked reduction (mrlbm_reduce_checked_noalloc). The
+// partial sums live in a file-scope device array, one copy per device for
+// the life of the process, instead of the context's partial buffer; the
+// kernels and the block count are those of mrlbm_reduce_checked, so the
+// result is the same bit for bit. In host code `scratch` names a host-side
+// stand-in: kernels get the device address that hipGetSymbolAddress returns
+// for the current device, looked up once per device and kept.
+constexpr int NOALLOC_RED_BLOCKS = 1024;  // 1024 partial triples, 24 KiB
+static __device__ __align__(256) double scratch[3 * NOALLOC_RED_BLOCKS];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch;  // per device: device address of `scratch`
+}  // namespace
+
+int noalloc_scratch_setup(int device, double** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0) return fail(MRLBM_EINVAL, "device %d", device);
+    if ((int)g_scratch.size() <= device) g_scratch.resize((size_t)device + 1, nullptr);
+    if (!g_scratch[device]) {
+        int cur = -1;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != device) return fail(MRLBM_EHIP, "scratch set-up for device %d while device %d is current", device, cur);
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MRLBM_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        g_scratch[device] = (double*)p;
+    }
+    if (out) *out = g_scratch[device];
+    return MRLBM_OK;
+}
+
 }  // namespace mrlbm
+
+extern "C" int mrlbm_reduce_checked_noalloc(mrlbm_ctx* c, double* d_out3) {
+    using namespace mrlbm;
+    if (!c || !d_out3) return fail(MRLBM_EINVAL, "null argument");
+    if (c->red_blocks > NOALLOC_RED_BLOCKS)
+        return fail(MRLBM_EINVAL, "reduction uses %d partial blocks, the scratch holds %d", c->red_blocks,
+                    NOALLOC_RED_BLOCKS);
+    HIPCHK(hipSetDevice(c->p.device));
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, d_out3) != hipSuccess || at.type != hipMemoryTypeDevice || at.device != c->p.device) {
+        (void)hipGetLastError();  // clear the query's error state
+        return fail(MRLBM_EINVAL, "d_out3 must be device memory on device %d", c->p.device);
+    }
+    double* part = nullptr;
+    const int rc = noalloc_scratch_setup(c->p.device, &part);
+    if (rc) return rc;
+    HIPCHK(launch_reduce(c->p.storage, c->d_mom, c->ntiles, c->T, (float)c->p.rho0, c->mrscale[0], c->mrscale[1], part,
+                         c->red_blocks, d_out3, c->stream, (double)c->N));
+    return MRLBM_OK;
+}

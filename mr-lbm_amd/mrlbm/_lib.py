@@ -98,6 +98,7 @@ _SIGS = {
     "mrlbm_write_moments": [_vp, _dp],
     "mrlbm_reduce": [_vp, _dp, _dp],
     "mrlbm_reduce_checked": [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64)],
+    "mrlbm_reduce_checked_noalloc": [_vp, _vp],
     "mrlbm_velocity_residual": [_vp, _dp],
     "mrlbm_checkpoint_save": [_vp, ctypes.c_char_p],
     "mrlbm_checkpoint_load": [_vp, ctypes.c_char_p],
