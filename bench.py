@@ -236,11 +236,11 @@ def reduced_precision_variants(n, tau, u0, launches, warmup):
     return res
 
 
-def channel_d3q27(steps, shape=(512, 256, 256), g=1e-6, nu=0.05):
+def channel_d3q27(steps, shape=(512, 256, 256), g=1e-6, nu=0.05, storages=("fp32", "fp16", "bf16")):
     """D3Q27 force-driven duct (x periodic, walls on y and z) from rest at each
     storage precision: throughput and the x-averaged streamed velocity against
     the transient rectangular-duct series at t = steps (and against the
-    FP32-storage run)."""
+    first storage's run, FP32 by default)."""
     import numpy as np
     import mrlbm
     from mrlbm import analytic, geometry
@@ -248,7 +248,9 @@ def channel_d3q27(steps, shape=(512, 256, 256), g=1e-6, nu=0.05):
     types = geometry.classify(shape, (0, 1, 1))
     ser = analytic.duct_start_up(shape[1], shape[2], g, nu, t=steps)
     out, ref = {}, None
-    for name, st in (("fp32", mrlbm.STORE_FP32), ("fp16", mrlbm.STORE_FP16), ("bf16", mrlbm.STORE_BF16)):
+    codes = {"fp32": mrlbm.STORE_FP32, "fp16": mrlbm.STORE_FP16, "bf16": mrlbm.STORE_BF16}
+    for name in storages:
+        st = codes[name]
         with mrlbm.Lattice(shape, q=27, tau=0.5 + 3 * nu, walls=(0, 1, 1), force=(g, 0, 0), storage=st) as lat:
             lat.set_node_types(types)
             lat.init_uniform()
@@ -295,6 +297,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2000, help="steps of each lattice in the batching comparison")
     ap.add_argument("--channel-steps", type=int, default=20000, help="steps of the D3Q27 duct at each precision")
     ap.add_argument("--no-channel", action="store_true", help="skip the D3Q27 duct lines")
+    ap.add_argument("--channel-storage", default="fp32,fp16,bf16",
+                    help="comma-separated storage formats of the D3Q27 duct lines (FP32 first is the reference)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-cavity batching comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served duct stream")
     args = ap.parse_args()
@@ -400,7 +404,7 @@ def main():
         out["variants"] = reduced_precision_variants(n, tau, u0, max(200, args.steps), max(20, args.warmup))
     if not args.no_channel:
         phase("D3Q27 duct at FP32/FP16/BF16")
-        out["channel_d3q27"] = channel_d3q27(args.channel_steps)
+        out["channel_d3q27"] = channel_d3q27(args.channel_steps, storages=args.channel_storage.split(","))
     if not args.no_cpu_baseline:
         phase("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

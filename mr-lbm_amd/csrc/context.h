@@ -91,14 +91,18 @@ struct mrlbm_ctx {
         const double omega = 1.0 / p.tau;
         // relaxation constants as src/mlbm.cu:56-60 derives them
         const float om = (float)omega;
-        a.R.omega = om;
+        // storage scales of U and M folded into the constants (powers of two,
+        // 1 for FP32 and BF16: the products are exact rescalings)
+        const float su = mscale[1], sm = mscale[4];
+        const float uu = sm / (su * su);
+        a.R.omega = om * uu;
         a.R.t_omega = 1.0f - om;
-        a.R.tt_omega = 1.0f - om / 2.0f;
-        a.R.omega_d2 = om / 2.0f;
-        a.R.tt_omega_t3 = a.R.tt_omega * 3.0f;
-        a.R.fx = (float)p.fx;
-        a.R.fy = (float)p.fy;
-        a.R.fz = (float)p.fz;
+        a.R.tt_omega = (1.0f - om / 2.0f) * uu;
+        a.R.omega_d2 = (om / 2.0f) * uu;
+        a.R.tt_omega_t3 = ((1.0f - om / 2.0f) * 3.0f) * uu;
+        a.R.fx = (float)p.fx * su;
+        a.R.fy = (float)p.fy * su;
+        a.R.fz = (float)p.fz * su;
         a.R.has_force = (p.fx != 0.0 || p.fy != 0.0 || p.fz != 0.0) ? 1 : 0;
         for (int m = 0; m < 10; m++) {
             a.mscale[m] = mscale[m];

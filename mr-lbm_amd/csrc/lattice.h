@@ -212,14 +212,20 @@ __device__ __forceinline__ float recon_pop(const Recon& t) {
     }
 }
 
+// All Q populations from the terms (re0 and drho as recon_terms used them).
+template <int Q>
+__device__ __forceinline__ void reconstruct_t(const Recon& t, float re0, float drho, float* g) {
+    constexpr float w0 = (float)VS<Q>::wd(0);
+    g[0] = __builtin_fmaf(w0, re0, w0 * drho);
+    static_for<Q - 1>([&](auto I) { g[decltype(I)::value + 1] = recon_pop<Q, decltype(I)::value + 1>(t); });
+}
+
 template <int Q>
 __device__ __forceinline__ void reconstruct(const Node& n, float* g) {
-    constexpr float w0 = (float)VS<Q>::wd(0);
     const float re0 = recon_re0<Q>(n);
     Recon t;
     recon_terms<Q>(n, re0, t);
-    g[0] = __builtin_fmaf(w0, re0, w0 * n.drho);
-    static_for<Q - 1>([&](auto I) { g[decltype(I)::value + 1] = recon_pop<Q, decltype(I)::value + 1>(t); });
+    reconstruct_t<Q>(t, re0, n.drho, g);
 }
 
 // Moments of the streamed populations at a bulk node
@@ -227,8 +233,11 @@ __device__ __forceinline__ void reconstruct(const Node& n, float* g) {
 // then scaled by 3, 4.5, 9. From deviations: drho = sum g, j = sum g c,
 // m_aa = (sum g c_a^2 - drho/3) / rho, m_ab = sum g c_a c_b / rho. Sums are
 // formed from the pair sums p = g_i + g_i' and differences d = g_i - g_i'.
+// su, sm: power-of-two scales of the outputs U and M (the storage scales
+// of a 16-bit format, whose rounding then needs no further multiply); the
+// density deviation is not scaled.
 template <int Q>
-__device__ __forceinline__ void bulk_moments(const float* g, float rho0, Node& n) {
+__device__ __forceinline__ void bulk_moments(const float* g, float rho0, Node& n, float su = 1.0f, float sm = 1.0f) {
     const float p12 = g[1] + g[2], d12 = g[1] - g[2];
     const float p34 = g[3] + g[4], d34 = g[3] - g[4];
     const float p56 = g[5] + g[6], d56 = g[5] - g[6];
@@ -267,7 +276,7 @@ __device__ __forceinline__ void bulk_moments(const float* g, float rho0, Node& n
     n.rho = rho0 + r;
     const float inv = __builtin_amdgcn_rcpf(n.rho);
     n.inv = inv;
-    const float u3 = 3.0f * inv, m45 = 4.5f * inv, m9 = 9.0f * inv;
+    const float u3 = (3.0f * su) * inv, m45 = (4.5f * sm) * inv, m9 = (9.0f * sm) * inv;
     const float r3 = r * (1.0f / 3.0f);
     n.ux = jx * u3;
     n.uy = jy * u3;
@@ -356,19 +365,26 @@ __device__ __forceinline__ void bc_solve(P c, const float* g, float rho0, Node& 
     n.mzz = (c[67] + dy[6]) * iy0;
 }
 
-// Scale unscaled u, m to the stored convention (src/mlbm.cu:500-509).
-__device__ __forceinline__ void scale_moments(Node& n) {
-    n.ux = 3.0f * n.ux;
-    n.uy = 3.0f * n.uy;
-    n.uz = 3.0f * n.uz;
-    n.mxx = 4.5f * n.mxx;
-    n.mxy = 9.0f * n.mxy;
-    n.mxz = 9.0f * n.mxz;
-    n.myy = 4.5f * n.myy;
-    n.myz = 9.0f * n.myz;
-    n.mzz = 4.5f * n.mzz;
+// Scale unscaled u, m to the stored convention (src/mlbm.cu:500-509),
+// times the power-of-two scales su (U) and sm (M) of bulk_moments.
+__device__ __forceinline__ void scale_moments(Node& n, float su = 1.0f, float sm = 1.0f) {
+    n.ux = (3.0f * su) * n.ux;
+    n.uy = (3.0f * su) * n.uy;
+    n.uz = (3.0f * su) * n.uz;
+    n.mxx = (4.5f * sm) * n.mxx;
+    n.mxy = (9.0f * sm) * n.mxy;
+    n.mxz = (9.0f * sm) * n.mxz;
+    n.myy = (4.5f * sm) * n.myy;
+    n.myz = (9.0f * sm) * n.myz;
+    n.mzz = (4.5f * sm) * n.mzz;
 }
 
+// Relaxation constants of collide(). With U and M carried times
+// power-of-two scales su and sm (the 16-bit storages' scales, 1 for FP32
+// and BF16), the host folds the scales in: the force fx, fy, fz carries su
+// (it enters U), and omega, omega_d2, tt_omega, tt_omega_t3 carry sm / su^2
+// (each multiplies a product of two U-scaled values into a second moment).
+// Every product is then the unscaled one times a power of two, exactly.
 struct Relax {
     float omega, t_omega, tt_omega, omega_d2, tt_omega_t3;
     float fx, fy, fz;

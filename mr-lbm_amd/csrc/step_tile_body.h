@@ -242,8 +242,12 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gou
     if constexpr (HAS_BC) s_cls[t] = (uint8_t)cls;
     const bool solid = HAS_BC && (cls == 255u);
 
+    // FP16: the reconstruction reads the fetched binary16 words directly
+    // (recon_terms_f16); FP32, BF16: decoded moments
+    constexpr bool F16 = S == 1;
+    constexpr float SU = store_scale(S, 1), SM = store_scale(S, 4);
     Node n;
-    decode_node<S>(a, in.m, n);
+    if constexpr (!F16) decode_node<S>(a, in.m, n);
     // populations entering the tile: ghost chunks -> LDS halo
     dispatch<NW>(w, [&](auto WW) {
         constexpr int W = decltype(WW)::value;
@@ -262,7 +266,14 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gou
         return tx + P::dx(i) * (ty + P::dy(i) * tz);
     };
     float g[Q];
-    reconstruct<Q>(n, g);
+    if constexpr (F16) {
+        Recon t0;
+        float re0;
+        recon_terms_f16<Q, false>(in.m, a.rho0, n.drho, n.rho, re0, t0);
+        reconstruct_t<Q>(t0, re0, n.drho, g);
+    } else {
+        reconstruct<Q>(n, g);
+    }
     static_for<Q - 1>([&](auto I) {
         constexpr int i = decltype(I)::value + 1;
         if constexpr (!P::x_only(i)) {
@@ -298,9 +309,9 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gou
     Node m;
     if (HAS_BC && cls != 0u && cls != 255u) {
         bc_solve<Q>(gptr(a.bc) + (size_t)cls * BC_STRIDE, g, a.rho0, m);
-        scale_moments(m);
+        scale_moments(m, SU, SM);
     } else {
-        bulk_moments<Q>(g, a.rho0, m);
+        bulk_moments<Q>(g, a.rho0, m, SU, SM);
     }
     collide(m, a.R);
     typename Codec<S>::T q[10];
@@ -311,9 +322,16 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gou
     const uint32_t nxu = (uint32_t)a.nx, nxy = nxu * (uint32_t)a.ny;
     const uint32_t zw = (uint32_t)(bz * BZ) + (uint32_t)__builtin_amdgcn_readfirstlane(tz);
     const uint32_t node_id = ((uint32_t)(bx * BX) + nxu * (uint32_t)(by * BY) + nxy * zw) + ((uint32_t)tx + __umul24(nxu, (uint32_t)ty));
-    quantize<S>(m, a, step, node_id, q);
     Recon rt;
-    recon_terms<Q>(m, recon_re0<Q>(m), rt);
+    if constexpr (F16) {
+        uint32_t pk[5];
+        quantize<S>(m, a, step, node_id, q, pk);
+        float re0;
+        recon_terms_f16<Q, true>(pk, a.rho0, m.drho, m.rho, re0, rt);
+    } else {
+        quantize<S>(m, a, step, node_id, q);
+        recon_terms<Q>(m, recon_re0<Q>(m), rt);
+    }
     const size_t mb = (size_t)tile * 10 * T + t;
     if (!solid) store_node<Q, BX, BY, BZ, S>(a, mb, q);
 
@@ -447,10 +465,20 @@ __global__ __launch_bounds__(BX* BY* BZ) void k_init_ghosts(StepArgs a, GhostOut
     const int t = threadIdx.x;
     const int tx = t % BX, ty = (t / BX) % BY, tz = t / (BX * BY);
     const int tile = blockIdx.x;
-    Node n;
-    load_node<Q, BX, BY, BZ, S>(a, (size_t)tile * 10 * T + t, n);
     float pop[Q];
-    reconstruct<Q>(n, pop);
+    if constexpr (S == 1) {  // as the step kernels reconstruct from fetched binary16 words
+        const auto mom = gptr((const uint16_t*)a.mom);
+        uint32_t w[10];
+        for (int k = 0; k < 10; k++) w[k] = mom[(size_t)tile * 10 * T + (size_t)k * T + t];
+        Recon rt;
+        float drho, rho, re0;
+        recon_terms_f16<Q, false>(w, a.rho0, drho, rho, re0, rt);
+        reconstruct_t<Q>(rt, re0, drho, pop);
+    } else {
+        Node n;
+        load_node<Q, BX, BY, BZ, S>(a, (size_t)tile * 10 * T + t, n);
+        reconstruct<Q>(n, pop);
+    }
     if (tx == 0) put_face_c<Q, BX, BY, BZ, 0>(g.f[0], ty, tz, tile, pop);
     if (tx == BX - 1) put_face_c<Q, BX, BY, BZ, 1>(g.f[1], ty, tz, tile, pop);
     if (ty == 0) put_face_c<Q, BX, BY, BZ, 2>(g.f[2], tx, tz, tile, pop);

@@ -85,9 +85,102 @@ __device__ __forceinline__ uint32_t sr_operand(uint32_t w0, uint32_t w1, uint32_
     else return w2;
 }
 
+// Reconstruction terms straight from binary16 moment words (storage format
+// 1): word k holds moment k in its low half (fetched words, PACKED false),
+// or moments 2j and 2j+1 in the low and high halves of word j (quantized
+// pairs, PACKED true). Each product of a term with a stored moment is one
+// v_fma_mix_f32 on the binary16 value, which it converts exactly, with the
+// moment's power-of-two storage scale folded into the other factor; sums
+// of stored moments are formed the same way. Scaling by a power of two
+// commutes with FP32 rounding, so the terms, re0 and drho are bit for bit
+// those recon_terms / recon_re0 give for the decoded moments: the step
+// kernels and the ghost initialisation form the same populations whichever
+// of the two they use.
+template <bool HI>
+__device__ __forceinline__ float mix_mul(float a, uint32_t b) {  // a * b, one rounding (b: binary16 half)
+    float r;
+    const float nz = -0.0f;  // addend -0: the result is the product itself, signed zero included
+    if constexpr (HI)
+        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[0,1,0] op_sel_hi:[0,1,0]" : "=v"(r) : "v"(a), "v"(b), "v"(nz));
+    else
+        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,1,0]" : "=v"(r) : "v"(a), "v"(b), "v"(nz));
+    return r;
+}
+template <bool HA, bool HB>
+__device__ __forceinline__ float mix_add(uint32_t a, uint32_t b) {  // a + b of two binary16 halves
+    float r;
+    if constexpr (!HA && !HB)
+        asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "v"(b));
+    else if constexpr (HA && !HB)
+        asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "v"(b));
+    else if constexpr (!HA && HB)
+        asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel:[0,0,1] op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "v"(b));
+    else
+        asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel:[1,0,1] op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+template <bool HB>
+__device__ __forceinline__ float mix_addf(float x, uint32_t b) {  // x + b, b a binary16 half
+    float r;
+    if constexpr (HB)
+        asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(b), "v"(x));
+    else
+        asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(b), "v"(x));
+    return r;
+}
+
+template <int Q, bool PACKED>
+__device__ __forceinline__ void recon_terms_f16(const uint32_t* w, float rho0, float& drho, float& rho, float& re0,
+                                                Recon& t) {
+    constexpr float sd = store_scale(1, 0), su = store_scale(1, 1), sm = store_scale(1, 4);
+    static_assert(store_scale(1, 2) == su && store_scale(1, 3) == su && store_scale(1, 9) == sm, "one scale per group");
+    constexpr float w1 = (float)VS<Q>::wd(1), w2 = (float)VS<Q>::wd(7);
+    auto W = [&](auto K) { return PACKED ? w[decltype(K)::value >> 1] : w[decltype(K)::value]; };
+    auto mul = [&](float a, auto K) {
+        return mix_mul<PACKED && (decltype(K)::value & 1)>(a, W(K));
+    };
+    using std::integral_constant;
+    const integral_constant<int, 0> k0;
+    const integral_constant<int, 1> k1;
+    const integral_constant<int, 2> k2;
+    const integral_constant<int, 3> k3;
+    const integral_constant<int, 4> k4;
+    const integral_constant<int, 5> k5;
+    const integral_constant<int, 6> k6;
+    const integral_constant<int, 7> k7;
+    const integral_constant<int, 8> k8;
+    const integral_constant<int, 9> k9;
+    drho = mul(1.0f / sd, k0);
+    rho = rho0 + drho;
+    const float ru = rho * (1.0f / su), rm = rho * (1.0f / sm);
+    // (mxx + myy) + mzz in the storage scale, then as recon_re0
+    const float s2 = mix_add<PACKED && (4 & 1), PACKED && (7 & 1)>(W(k4), W(k7));
+    const float s3 = mix_addf<PACKED && (9 & 1)>(s2, W(k9));
+    re0 = rm * (-(1.0f / 3.0f) * s3);
+    t.rux = mul(ru, k1);
+    t.ruy = mul(ru, k2);
+    t.ruz = mul(ru, k3);
+    t.ex = re0 + mul(rm, k4);
+    t.ey = re0 + (t.ryy = mul(rm, k7));
+    t.ez = re0 + (t.rzz = mul(rm, k9));
+    t.rxy = mul(rm, k5);
+    t.rxz = mul(rm, k6);
+    t.ryz = mul(rm, k8);
+    t.d1 = mul(w1 / sd, k0);
+    t.d2 = mul(w2 / sd, k0);
+    if constexpr (Q == 27) t.d3 = mul((float)VS<Q>::wd(26) / sd, k0);
+    else t.d3 = 0.0f;
+}
+
+// FP16 storage: the moments U and M arrive already times their storage
+// scales (bulk_moments / scale_moments / collide carry them), so only the
+// density deviation is multiplied before rounding; pk[j] receives the
+// rounded pair (2j, 2j+1) for recon_terms_f16, and n.drho, n.rho the
+// stored density. FP32 and BF16 (scale 1): n is replaced by the stored
+// values.
 template <int S>
 __device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t step, uint32_t node_id,
-                                         typename Codec<S>::T* q) {
+                                         typename Codec<S>::T* q, uint32_t* pk = nullptr) {
     if constexpr (S == 0) {
         q[0] = n.drho;
         q[1] = n.ux;
@@ -110,13 +203,12 @@ __device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t st
             const uint32_t r0 = sr_operand<k>(w0, w1, w2), r1 = sr_operand<k + 1>(w0, w1, w2);
             uint32_t b;
             if constexpr (S == 1) {
-                constexpr float s0 = store_scale(1, k), s1 = store_scale(1, k + 1);
                 half2_t h = {};
-                h = __builtin_amdgcn_cvt_sr_f16_f32(h, *v[k] * s0, r0, false);
-                h = __builtin_amdgcn_cvt_sr_f16_f32(h, *v[k + 1] * s1, r1, true);
+                const float x0 = k == 0 ? *v[k] * store_scale(1, 0) : *v[k];  // U, M: already scaled
+                h = __builtin_amdgcn_cvt_sr_f16_f32(h, x0, r0, false);
+                h = __builtin_amdgcn_cvt_sr_f16_f32(h, *v[k + 1], r1, true);
                 b = __builtin_bit_cast(uint32_t, h);
-                *v[k] = f16_scaled<false>(b, 1.0f / s0);
-                *v[k + 1] = f16_scaled<true>(b, 1.0f / s1);
+                pk[k / 2] = b;
             } else {
                 bf162_t h = {};
                 h = __builtin_amdgcn_cvt_sr_bf16_f32(h, *v[k], r0, false);
@@ -128,7 +220,7 @@ __device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t st
             q[k] = (uint16_t)(b & 0xFFFFu);
             q[k + 1] = (uint16_t)(b >> 16);
         });
-        n.rho = a.rho0 + n.drho;
+        if constexpr (S == 2) n.rho = a.rho0 + n.drho;
     }
 }
 
