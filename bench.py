@@ -37,25 +37,37 @@ def traffic_from_profiles(workload_key):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(target_s):
-    """FP64 oracle (OpenMP over the host cores) on a TGV 64^3 sample of the
-    same workload, sized to about target_s seconds."""
+def cpu_baseline(target_s, n=256):
+    """FP64 oracle on the host cores, on the benchmark workload itself (TGV
+    n^3, D3Q19, tau 0.53): OpenMP over all threads for about target_s
+    seconds, then one thread for about a third of that (the serial figure)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle
 
-    n = 64
-    threads = _oracle.lib().orc_get_threads()
+    L = _oracle.lib()
+    threads = L.orc_get_threads()
     o = _oracle.Oracle(n, n, n, q=19, tau=0.53)
     o.init_taylor_green(0.05)
-    t1 = o.step_timed(20)  # calibration (also absorbs the thread pool's start-up)
-    steps = max(2, min(5000, int(target_s / max(t1 / 20, 1e-6))))
+    t1 = o.step_timed(1)  # calibration (also absorbs the thread pool's start-up)
+    steps = max(2, min(5000, int(target_s / max(t1, 1e-6))))
     t = o.step_timed(steps)
+    L.orc_set_threads(1)
+    try:
+        t1s = o.step_timed(1)
+        steps_s = max(2, int(target_s / 3 / max(t1s, 1e-6)))
+        ts = o.step_timed(steps_s)
+    finally:
+        L.orc_set_threads(threads)
+    nodes = n ** 3
     return {
-        "value": n ** 3 * steps / t / 1e6,
+        "value": nodes * steps / t / 1e6,
         "unit": "MLUPS",
         "cores": threads,
         "kind": "port",
-        "sample": f"FP64 oracle, D3Q19 TGV {n}^3, {steps} steps ({t:.1f} s), OpenMP {threads} threads",
+        "sample": f"FP64 oracle, D3Q19 TGV {n}^3, {steps} steps ({t:.1f} s), "
+                  f"OpenMP {threads} threads",
+        "serial_mlups": nodes * steps_s / ts / 1e6,
+        "serial_sample": f"same lattice, {steps_s} steps ({ts:.1f} s), 1 thread",
     }
 
 

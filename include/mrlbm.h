@@ -58,7 +58,10 @@ typedef struct mrlbm_ctx mrlbm_ctx;
 typedef struct {
     int nx, ny, nz;        /* NX, NY, NZ                                        */
     int q;                 /* 19 (D3Q19) or 27 (D3Q27)                          */
-    int bx, by, bz;        /* tile shape; 0,0,0 = library default (8x8x8 tiles) */
+    int bx, by, bz;        /* tile shape (8x8x8, 16x8x4, 32x4x4, 64x4x2, 16x8x8;
+                              16x16x32, 16x16x64 with the column kernel); 0,0,0 =
+                              library default: 16x8x8 for FP32 D3Q19 where it
+                              divides the lattice, else 8x8x8 */
     int storage;           /* MRLBM_STORE_*                                     */
     int wall_x, wall_y, wall_z; /* 0 periodic (BC_*_PERIODIC), 1 wall (BC_*_WALL) */
     int kernel;            /* MRLBM_KERNEL_*                                    */

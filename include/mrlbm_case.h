@@ -97,12 +97,18 @@ constexpr int WALL_Z = 1;
 constexpr int WALL_Z = 0;
 #endif
 
-/* Lattice parameters of the case (tile = library default, device 0). */
+/* Lattice parameters of the case (device 0). The tile is the reference's
+ * block, BLOCK_NX x BLOCK_NY x BLOCK_NZ = 8x8x8 (src/definitions.h:90-92):
+ * a case whose node types read populations from outside the domain runs
+ * as the reference's kernel runs it only at that block shape. */
 inline mrlbm_params params(int storage = MRLBM_STORE_FP32, int device = 0) {
     mrlbm_params p{};
     p.nx = NX;
     p.ny = NY;
     p.nz = NZ_TOTAL;
+    p.bx = 8;
+    p.by = 8;
+    p.bz = 8;
     p.q = Q;
     p.storage = storage;
     p.wall_x = WALL_X;

@@ -93,10 +93,16 @@ int mrlbm_create(const mrlbm_params* pp, mrlbm_ctx** out) {
     int bx = p.bx, by = p.by, bz = p.bz;
     int kernel = p.kernel;
     if (bx == 0 && by == 0 && bz == 0) {
-        // default: 8x8x8 tiles (the fastest measured shape), or for the
-        // column kernel the longest segment that divides the lattice
-        const int cand[3][3] = {{8, 8, 8}, {16, 16, 32}, {16, 16, 64}};
-        for (int c = 0; c < 3; c++) {
+        // default: the fastest measured tile that divides the lattice, or for
+        // the column kernel the longest segment that does. FP32 D3Q19 is
+        // HBM-bound, and 16x8x8 tiles move 105 instead of 110 B per update
+        // (256^3 TGV 54.9 -> 57.7 GLUPS, cavity 50.9 -> 51.3); the 16-bit
+        // storages and D3Q27 are bound by instruction issue and latency,
+        // which the 1024-thread tiles' two workgroups per CU hide worse
+        // (FP16 TGV 74.9 -> 68.0 GLUPS), so they keep 8x8x8.
+        const bool big = p.q == 19 && p.storage == MRLBM_STORE_FP32;
+        const int cand[4][3] = {{16, 8, 8}, {8, 8, 8}, {16, 16, 32}, {16, 16, 64}};
+        for (int c = big ? 0 : 1; c < 4; c++) {
             const bool col = cand[c][0] * cand[c][1] == 256;
             if (kernel != MRLBM_KERNEL_COLUMN && col) continue;
             if (kernel == MRLBM_KERNEL_COLUMN && !col) continue;

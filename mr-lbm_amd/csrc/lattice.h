@@ -331,7 +331,7 @@ __device__ __forceinline__ void bc_solve(P c, const float* g, float rho0, Node& 
         if (cyv(i) * czv(i) < 0) syz = syz - f;
     }
     const float rI = rho0 * c[53] + r;
-    const float inv = 1.0f / rI;
+    const float inv = __builtin_amdgcn_rcpf(rI);  // v_rcp_f32 (1 ulp): the BC moments' rounding level
     const float r3 = r * (1.0f / 3.0f);
     float db[6];
     db[0] = ((sxx - r3) - c[54] * r) * inv;
@@ -349,7 +349,7 @@ __device__ __forceinline__ void bc_solve(P c, const float* g, float rho0, Node& 
         dy[k] = acc;
     }
     const float y0 = c[61] + dy[0];
-    const float iy0 = 1.0f / y0;
+    const float iy0 = __builtin_amdgcn_rcpf(y0);
     // rho - RHO_0 = rho_I y0 - RHO_0 = RHO_0 d0 + r y_rest,0 + rho_I dy0
     n.drho = (rho0 * c[60] + r * c[61]) + rI * dy[0];
     n.rho = rho0 + n.drho;

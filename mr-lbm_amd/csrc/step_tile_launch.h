@@ -11,7 +11,8 @@
     X(8, 8, 8)               \
     X(16, 8, 4)              \
     X(32, 4, 4)              \
-    X(64, 4, 2)
+    X(64, 4, 2)              \
+    X(16, 8, 8)
 
 namespace mrlbm {
 
