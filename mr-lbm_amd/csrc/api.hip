@@ -67,7 +67,7 @@ int mrlbm_destroy(mrlbm_ctx* c) {
     hipFree(c->d_tiles);
     hipFree(c->d_mean);
     for (int s = 0; s < 2; s++)
-        for (int f = 0; f < 6; f++) hipFree(c->d_ghost[s][f]);
+        hipFree(c->d_gbase[s]);
     hipFree(c->d_partial);
     hipFree(c->d_red);
     hipFree(c->d_uprev);
@@ -174,12 +174,19 @@ int mrlbm_create(const mrlbm_params* pp, mrlbm_ctx** out) {
     c->face_elems[0] = c->face_elems[1] = (size_t)by * bz * c->qf * c->ntiles;
     c->face_elems[2] = c->face_elems[3] = (size_t)bx * bz * c->qf * c->ntiles;
     c->face_elems[4] = c->face_elems[5] = (size_t)bx * by * c->qf * c->ntiles;
-    for (int s = 0; s < 2; s++)
-        for (int f = 0; f < 6; f++)
-            if ((rc = alloc((void**)&c->d_ghost[s][f], c->face_elems[f] * sizeof(float)))) {
-                mrlbm_destroy(c);
-                return rc;
-            }
+    for (int s = 0; s < 2; s++) {
+        size_t set = 0;
+        for (int f = 0; f < 6; f++) set += c->face_elems[f];
+        if ((rc = alloc((void**)&c->d_gbase[s], set * sizeof(float)))) {
+            mrlbm_destroy(c);
+            return rc;
+        }
+        size_t o = 0;
+        for (int f = 0; f < 6; f++) {
+            c->d_ghost[s][f] = c->d_gbase[s] + o;
+            o += c->face_elems[f];
+        }
+    }
     c->red_blocks = 1024;
     if ((rc = alloc((void**)&c->d_partial, sizeof(double) * 3 * c->red_blocks)) ||
         (rc = alloc((void**)&c->d_red, sizeof(double) * 4))) {

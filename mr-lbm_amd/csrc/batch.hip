@@ -70,7 +70,7 @@ int refresh(mrlbm_batch* b) {
         InstArgs ia{};
         ia.a = c->args();
         for (int s = 0; s < 2; s++)
-            for (int f = 0; f < 6; f++) ia.g[s][f] = c->d_ghost[s][f];
+            ia.g[s] = c->d_gbase[s];
         ia.cur0 = c->cur;
         ia.step0 = (uint32_t)c->step;
         ia.has_bc = c->has_bc ? 1 : 0;

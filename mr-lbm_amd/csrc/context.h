@@ -34,7 +34,8 @@ struct mrlbm_ctx {
     uint8_t* d_cls = nullptr;
     float* d_bc = nullptr;
     size_t bc_bytes = 0;  // capacity of d_bc
-    float* d_ghost[2][6] = {};
+    float* d_gbase[2] = {};       // ghost sets, six faces one after another (ghost_face)
+    float* d_ghost[2][6] = {};    // the faces inside them
     size_t face_elems[6] = {};
     int cur = 0;  // ghost set read by the next step
     int64_t step = 0;
@@ -74,10 +75,9 @@ struct mrlbm_ctx {
         a.mom = d_mom;
         a.cls = d_cls;
         a.bc = d_bc;
-        for (int f = 0; f < 6; f++) {
-            a.gin[f] = d_ghost[cur][f];
-            a.gout[f] = d_ghost[1 - cur][f];
-        }
+        a.gin = d_gbase[cur];
+        a.gout = d_gbase[1 - cur];
+        a.ntiles = ntiles;
         a.ntx = ntx;
         a.nty = nty;
         a.ntz = ntz;

@@ -57,22 +57,72 @@ struct HaloGeom {
 // LDS arrays of the populations: extents B_a + |c_a| (the entering plane on
 // the upwind side), or the entering x plane only for populations that move
 // along x alone when BX = 8 or 16 (DPP lane shifts carry the rest).
+template <int BX>
+__host__ __device__ constexpr bool pop_x_only(int i) {
+    return (BX == 8 || BX == 16) && cyv(i) == 0 && czv(i) == 0 && cxv(i) != 0;
+}
+template <int BX, int BY, int BZ>
+__host__ __device__ constexpr int pop_cells(int i) {
+    return (pop_x_only<BX>(i) ? 1 : BX + (cxv(i) != 0)) * (BY + (cyv(i) != 0)) * (BZ + (czv(i) != 0));
+}
+
+// Where each population's array lives. D3Q27 streams its populations
+// through LDS in two phases that reuse the same space (two barriers more),
+// so that its arrays take about half the LDS: 8x8x8 fits four workgroups per
+// CU instead of two. Opposite pairs (2k-1, 2k) go, largest first, to the
+// phase holding fewer cells so far. D3Q19 streams in one phase.
+struct PopLayout {
+    int phase[27];
+    int off[27];    // array offset within its phase's space
+    int total[2];   // cells of each phase
+};
+template <int Q, int BX, int BY, int BZ>
+constexpr PopLayout make_pop_layout() {
+    PopLayout L{};
+    constexpr int PHASES = Q == 27 ? 2 : 1;
+    if (PHASES == 2) {
+        bool done[14] = {};
+        for (int n = 0; n < (Q - 1) / 2; n++) {
+            int best = 0, bs = -1;
+            for (int k = 1; k <= (Q - 1) / 2; k++) {
+                const int sz = pop_cells<BX, BY, BZ>(2 * k - 1) + pop_cells<BX, BY, BZ>(2 * k);
+                if (!done[k] && sz > bs) bs = sz, best = k;
+            }
+            done[best] = true;
+            const int p = L.total[0] <= L.total[1] ? 0 : 1;
+            L.total[p] += bs;
+            L.phase[2 * best - 1] = L.phase[2 * best] = p;
+        }
+    }
+    int used[2] = {0, 0};
+    for (int i = 1; i < Q; i++) {
+        L.off[i] = used[L.phase[i]];
+        used[L.phase[i]] += pop_cells<BX, BY, BZ>(i);
+    }
+    L.total[0] = used[0];
+    L.total[1] = used[1];
+    return L;
+}
+template <int Q, int BX, int BY, int BZ>
+inline constexpr PopLayout pop_layout_v = make_pop_layout<Q, BX, BY, BZ>();
+
 template <int Q, int BX, int BY, int BZ>
 struct PopLds {
     static constexpr bool DPP_X = BX == 8 || BX == 16;
-    __host__ __device__ static constexpr bool x_only(int i) {
-        return DPP_X && cyv(i) == 0 && czv(i) == 0 && cxv(i) != 0;
-    }
+    static constexpr int PHASES = Q == 27 ? 2 : 1;
+    __host__ __device__ static constexpr bool x_only(int i) { return pop_x_only<BX>(i); }
     __host__ __device__ static constexpr int dx(int i) { return x_only(i) ? 1 : BX + (cxv(i) != 0); }
     __host__ __device__ static constexpr int dy(int i) { return BY + (cyv(i) != 0); }
     __host__ __device__ static constexpr int dz(int i) { return BZ + (czv(i) != 0); }
     __host__ __device__ static constexpr int size(int i) { return dx(i) * dy(i) * dz(i); }
-    __host__ __device__ static constexpr int off(int i) {
-        int o = 0;
-        for (int j = 1; j < i; j++) o += size(j);
-        return o;
+    __host__ __device__ static constexpr int phase(int i) { return pop_layout_v<Q, BX, BY, BZ>.phase[i]; }
+    // offset of population i's array within its phase's space
+    __host__ __device__ static constexpr int off(int i) { return pop_layout_v<Q, BX, BY, BZ>.off[i]; }
+    // cells of the largest phase (the LDS the arrays need)
+    __host__ __device__ static constexpr int total() {
+        constexpr PopLayout L = pop_layout_v<Q, BX, BY, BZ>;
+        return L.total[0] > L.total[1] ? L.total[0] : L.total[1];
     }
-    __host__ __device__ static constexpr int total() { return off(Q); }
     // stride of array axis a
     __host__ __device__ static constexpr int stride(int i, int a) {
         return a == 0 ? (x_only(i) ? 0 : 1) : (a == 1 ? dx(i) : dx(i) * dy(i));

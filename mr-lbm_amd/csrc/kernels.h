@@ -101,8 +101,9 @@ struct StepArgs {
     void* mom;               // 10 moment planes per tile, block-major SoA
     const uint8_t* cls;      // node class per node (block layout) or null
     const float* bc;         // boundary class table (BC_STRIDE floats each)
-    const float* gin[6];
-    float* gout[6];
+    const float* gin;        // ghost set read: the six faces one after another (ghost_face)
+    float* gout;             // ghost set written
+    int ntiles;
     int ntx, nty, ntz;       // tiles per axis
     int nx, ny, nz;
     int wall_x, wall_y, wall_z;
@@ -127,6 +128,25 @@ __host__ __device__ __forceinline__ size_t gidx_z(int tx, int ty, int slot, int 
     return (size_t)tx + BX * ((size_t)ty + BY * ((size_t)slot + (size_t)QF * tile));
 }
 
+// A ghost set holds the six faces one after another: face f keeps
+// face_tile_cells(f) values per tile (slot-major per tile, tiles x-fastest,
+// the reference's idxPopX/Y/Z order within the face). One base pointer per
+// set keeps the step kernels' scalar registers down (12 fewer than six
+// face pointers per set).
+__host__ __device__ constexpr size_t face_tile_cells(int f, int bx, int by, int bz, int qf) {
+    return (size_t)(f < 2 ? by * bz : (f < 4 ? bx * bz : bx * by)) * (size_t)qf;
+}
+__host__ __device__ constexpr size_t face_tile_offset(int f, int bx, int by, int bz, int qf) {
+    size_t o = 0;
+    for (int j = 0; j < f; j++) o += face_tile_cells(j, bx, by, bz, qf);
+    return o;
+}
+// Face f of the ghost set at `base` (compile-time tile shape).
+template <int BX, int BY, int BZ, int QF, typename T>
+__host__ __device__ __forceinline__ T* ghost_face(T* base, int f, int ntiles) {
+    return base + face_tile_offset(f, BX, BY, BZ, QF) * (size_t)ntiles;
+}
+
 struct GhostOut {
     float* f[6];
 };
@@ -136,7 +156,7 @@ struct GhostOut {
 // a sequence of n launches an advance kernel adds n to both.
 struct InstArgs {
     StepArgs a;      // gin/gout/step are filled per launch
-    float* g[2][6];  // both ghost sets
+    float* g[2];     // both ghost sets (base pointers)
     int cur0;
     uint32_t step0;
     int has_bc;

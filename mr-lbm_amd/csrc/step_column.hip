@@ -167,12 +167,12 @@ __global__ __launch_bounds__(BX* BY) void k_step_column(StepArgs a) {
         const bool solid = HAS_BC && (cls == 255u);
         if (!solid) {
             store_node<Q, BX, BY, BZ, S>(a, mbase + (size_t)k * P, qv);
-            if (tx == 0 && (!a.wall_x || x != 0)) put_face_c<Q, BX, BY, BZ, 0>(a.gout[0], ty, k, tile, g);
-            if (tx == BX - 1 && (!a.wall_x || x != a.nx - 1)) put_face_c<Q, BX, BY, BZ, 1>(a.gout[1], ty, k, tile, g);
-            if (ty == 0 && (!a.wall_y || y != 0)) put_face_c<Q, BX, BY, BZ, 2>(a.gout[2], tx, k, tile, g);
-            if (ty == BY - 1 && (!a.wall_y || y != a.ny - 1)) put_face_c<Q, BX, BY, BZ, 3>(a.gout[3], tx, k, tile, g);
-            if (k == 0 && (!a.wall_z || z != 0)) put_face_c<Q, BX, BY, BZ, 4>(a.gout[4], tx, ty, tile, g);
-            if (k == BZ - 1 && (!a.wall_z || z != a.nz - 1)) put_face_c<Q, BX, BY, BZ, 5>(a.gout[5], tx, ty, tile, g);
+            if (tx == 0 && (!a.wall_x || x != 0)) put_face_c<Q, BX, BY, BZ, 0>(ghost_face<BX, BY, BZ, VS<Q>::QF>(a.gout, 0, a.ntiles), ty, k, tile, g);
+            if (tx == BX - 1 && (!a.wall_x || x != a.nx - 1)) put_face_c<Q, BX, BY, BZ, 1>(ghost_face<BX, BY, BZ, VS<Q>::QF>(a.gout, 1, a.ntiles), ty, k, tile, g);
+            if (ty == 0 && (!a.wall_y || y != 0)) put_face_c<Q, BX, BY, BZ, 2>(ghost_face<BX, BY, BZ, VS<Q>::QF>(a.gout, 2, a.ntiles), tx, k, tile, g);
+            if (ty == BY - 1 && (!a.wall_y || y != a.ny - 1)) put_face_c<Q, BX, BY, BZ, 3>(ghost_face<BX, BY, BZ, VS<Q>::QF>(a.gout, 3, a.ntiles), tx, k, tile, g);
+            if (k == 0 && (!a.wall_z || z != 0)) put_face_c<Q, BX, BY, BZ, 4>(ghost_face<BX, BY, BZ, VS<Q>::QF>(a.gout, 4, a.ntiles), tx, ty, tile, g);
+            if (k == BZ - 1 && (!a.wall_z || z != a.nz - 1)) put_face_c<Q, BX, BY, BZ, 5>(ghost_face<BX, BY, BZ, VS<Q>::QF>(a.gout, 5, a.ntiles), tx, ty, tile, g);
         }
         // rotate the incoming arrays
         static_for<Q>([&](auto I) {

@@ -32,7 +32,7 @@ namespace mrlbm {
 // along each axis for offsets -1, 0, +1.
 struct Nbr {
     int x[3], y[3], z[3];
-    int ntx, nty;
+    int ntx, nty, ntiles;
     __device__ __forceinline__ int tile(int dx, int dy, int dz) const {
         return x[dx + 1] + ntx * (y[dy + 1] + nty * z[dz + 1]);
     }
@@ -44,7 +44,7 @@ struct Nbr {
 // them all. The address is the owner's plane base (uniform) + lane + a
 // constant: no per-lane index arithmetic.
 template <int Q, int BX, int BY, int BZ, int K>
-__device__ __forceinline__ float halo_fetch(const float* const* gin, int lane, const Nbr& nb) {
+__device__ __forceinline__ float halo_fetch(const float* gin, int lane, const Nbr& nb) {
     constexpr Chunk c = ChunkTables<Q, BX, BY, BZ>::in[K];
     constexpr int QF = VS<Q>::QF;
     constexpr int Bu = face_bu<BX, BY, BZ>(c.face);
@@ -52,7 +52,8 @@ __device__ __forceinline__ float halo_fetch(const float* const* gin, int lane, c
     constexpr int NC = Bu * c.rows;
     static_assert(64 % Bu == 0 && (NC & (NC - 1)) == 0, "face rows must tile a wave");
     const int otile = nb.tile(c.dx, c.dy, c.dz);
-    const auto src = gptr(gin[c.face]) + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)otile) + Bu * c.w0;
+    const auto src = gptr(ghost_face<BX, BY, BZ, QF>(gin, c.face, nb.ntiles)) +
+                     (size_t)(Bu * Bv) * (c.slot + QF * (size_t)otile) + Bu * c.w0;
     return src[NC >= 64 ? (uint32_t)lane : ((uint32_t)lane & (NC - 1))];
 }
 
@@ -80,7 +81,7 @@ __device__ __forceinline__ void halo_put(int lane, float v, float* s) {
 // from the reconstruction terms the node staged in LDS (term k of the node
 // at base nb is s[k * E + nb]) and writes it to the ghost plane.
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int K>
-__device__ __forceinline__ void halo_store(const StepArgs& a, float* const* gout, int lane, int tile, int bx, int by,
+__device__ __forceinline__ void halo_store(const StepArgs& a, float* gout, int lane, int tile, int bx, int by,
                                            int bz, const float* s, const uint8_t* s_cls) {
     constexpr Chunk c = ChunkTables<Q, BX, BY, BZ>::out[K];
     constexpr int QF = VS<Q>::QF;
@@ -97,7 +98,8 @@ __device__ __forceinline__ void halo_store(const StepArgs& a, float* const* gout
     if constexpr (c.face == 4) skip = a.wall_z && bz == 0;
     if constexpr (c.face == 5) skip = a.wall_z && bz == a.ntz - 1;
     if (skip) return;
-    const auto dst = gptr(gout[c.face]) + (size_t)(Bu * Bv) * (c.slot + QF * (size_t)tile) + Bu * c.w0;
+    const auto dst = gptr(ghost_face<BX, BY, BZ, QF>(gout, c.face, a.ntiles)) +
+                     (size_t)(Bu * Bv) * (c.slot + QF * (size_t)tile) + Bu * c.w0;
     int u, r;
     if (chunk_cell<Q, BX, BY, BZ, false, K>(lane, u, r)) {
         if constexpr (HAS_BC) {
@@ -169,13 +171,14 @@ __device__ __forceinline__ void tile_coords(const StepArgs& a, int tile, int& bx
     bz = tile / (a.ntx * a.nty);
     nb.ntx = a.ntx;
     nb.nty = a.nty;
+    nb.ntiles = a.ntiles;
     nb.x[0] = bx == 0 ? a.ntx - 1 : bx - 1, nb.x[1] = bx, nb.x[2] = bx == a.ntx - 1 ? 0 : bx + 1;
     nb.y[0] = by == 0 ? a.nty - 1 : by - 1, nb.y[1] = by, nb.y[2] = by == a.nty - 1 ? 0 : by + 1;
     nb.z[0] = bz == 0 ? a.ntz - 1 : bz - 1, nb.z[1] = bz, nb.z[2] = bz == a.ntz - 1 ? 0 : bz + 1;
 }
 
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
-__device__ __forceinline__ void tile_fetch(const StepArgs& a, const float* const* gin, int tile,
+__device__ __forceinline__ void tile_fetch(const StepArgs& a, const float* gin, int tile,
                                            TileIn<Q, BX, BY, BZ, S>& in) {
     constexpr int T = BX * BY * BZ;
     constexpr int NW = T / 64;
@@ -219,7 +222,7 @@ __device__ __forceinline__ void tile_fetch(const StepArgs& a, const float* const
 // with the same tile. `ahead` is called once the inputs are consumed
 // (before the first barrier): it issues the next tile's fetch.
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S, typename Ahead>
-__device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gout, uint32_t step, int tile,
+__device__ __forceinline__ void tile_update(const StepArgs& a, float* gout, uint32_t step, int tile,
                                             TileLds<Q, BX, BY, BZ, HAS_BC>& L, const TileIn<Q, BX, BY, BZ, S>& in,
                                             Ahead&& ahead) {
     constexpr int T = BX * BY * BZ;
@@ -248,19 +251,25 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gou
     constexpr float SU = store_scale(S, 1), SM = store_scale(S, 4);
     Node n;
     if constexpr (!F16) decode_node<S>(a, in.m, n);
-    // populations entering the tile: ghost chunks -> LDS halo
-    dispatch<NW>(w, [&](auto WW) {
-        constexpr int W = decltype(WW)::value;
-        constexpr int R = (NIN - W + NW - 1) / NW;
-        static_for<R>([&](auto RR) {
-            constexpr int r = decltype(RR)::value;
-            halo_put<Q, BX, BY, BZ, W + NW * r>(lane, in.h[r], s);
+    using P = PopLds<Q, BX, BY, BZ>;
+    constexpr int PH = P::PHASES;
+    // populations entering the tile in phase PP: ghost chunks -> LDS halo
+    auto halo_phase = [&](auto PP) {
+        dispatch<NW>(w, [&](auto WW) {
+            constexpr int W = decltype(WW)::value;
+            constexpr int R = (NIN - W + NW - 1) / NW;
+            static_for<R>([&](auto RR) {
+                constexpr int r = decltype(RR)::value;
+                constexpr int K = W + NW * r;
+                if constexpr (P::phase(ChunkTables<Q, BX, BY, BZ>::in[K].pop) == decltype(PP)::value)
+                    halo_put<Q, BX, BY, BZ, K>(lane, in.h[r], s);
+            });
         });
-    });
+    };
+    halo_phase(std::integral_constant<int, 0>());
 
     // population arrays: node base per (x extent, y extent) pair of the
     // arrays, so every access is a base register plus an immediate
-    using P = PopLds<Q, BX, BY, BZ>;
     auto pbase = [&](auto I) {
         constexpr int i = decltype(I)::value + 1;
         return tx + P::dx(i) * (ty + P::dy(i) * tz);
@@ -274,13 +283,6 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gou
     } else {
         reconstruct<Q>(n, g);
     }
-    static_for<Q - 1>([&](auto I) {
-        constexpr int i = decltype(I)::value + 1;
-        if constexpr (!P::x_only(i)) {
-            constexpr int wo = P::cell(i, cxv(i) > 0, cyv(i) > 0, czv(i) > 0);
-            s[pbase(I) + wo] = g[i];
-        }
-    });
     // populations moving along x only: from the x neighbour's lane (row_shr:1
     // for +x, row_shl:1 for -x: x rows lie inside 16-lane DPP rows); the
     // first node of a row takes the entering value from LDS after the barrier
@@ -292,18 +294,37 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* const* gou
             gx[i] = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(g[i]), ctrl, 0xF, 0xF, false));
         }
     });
-    ahead();
-    __syncthreads();
-    static_for<Q - 1>([&](auto I) {
-        constexpr int i = decltype(I)::value + 1;
-        if constexpr (P::x_only(i)) {
-            const bool edge = cxv(i) > 0 ? tx == 0 : tx == BX - 1;
-            const float h = s[P::off(i) + ty + BY * tz];
-            g[i] = edge ? h : gx[i];
-        } else {
-            constexpr int ro = P::cell(i, cxv(i) < 0, cyv(i) < 0, czv(i) < 0);
-            g[i] = s[pbase(I) + ro];
+    // each phase: write its populations, barrier, pull them (D3Q27 runs two
+    // phases through the same LDS, with a barrier between the first phase's
+    // reads and the second phase's writes)
+    static_for<PH>([&](auto PP) {
+        constexpr int ph = decltype(PP)::value;
+        if constexpr (ph > 0) {
+            __syncthreads();
+            halo_phase(PP);
         }
+        static_for<Q - 1>([&](auto I) {
+            constexpr int i = decltype(I)::value + 1;
+            if constexpr (!P::x_only(i) && P::phase(i) == ph) {
+                constexpr int wo = P::cell(i, cxv(i) > 0, cyv(i) > 0, czv(i) > 0);
+                s[pbase(I) + wo] = g[i];
+            }
+        });
+        if constexpr (ph == 0) ahead();
+        __syncthreads();
+        static_for<Q - 1>([&](auto I) {
+            constexpr int i = decltype(I)::value + 1;
+            if constexpr (P::phase(i) == ph) {
+                if constexpr (P::x_only(i)) {
+                    const bool edge = cxv(i) > 0 ? tx == 0 : tx == BX - 1;
+                    const float h = s[P::off(i) + ty + BY * tz];
+                    g[i] = edge ? h : gx[i];
+                } else {
+                    constexpr int ro = P::cell(i, cxv(i) < 0, cyv(i) < 0, czv(i) < 0);
+                    g[i] = s[pbase(I) + ro];
+                }
+            }
+        });
     });
 
     Node m;

@@ -257,7 +257,7 @@ template <int Q, int BX, int BY, int BZ, int FACE>
 __device__ __forceinline__ void load_face(const StepArgs& a, int tx, int ty, int tz, int bx, int by, int bz,
                                           float* vals) {
     constexpr int QF = VS<Q>::QF;
-    const auto g = gptr(a.gin[FACE]);
+    const auto g = gptr(ghost_face<BX, BY, BZ, QF>(a.gin, FACE, a.ntiles));
     static_for<QF>([&](auto SI) {
         constexpr int s = decltype(SI)::value;
         constexpr int p = face_pop(FACE, s);

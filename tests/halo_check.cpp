@@ -3,8 +3,11 @@
 //
 // LDS geometry, restated here independently of halo.h: population i has an
 // array of extents (B_a + |c_ia|) per axis, stored one after another in
-// population order, x fastest; when BX is 8 or 16, a population moving
-// along x only keeps just its entering x plane (extent 1 along x).
+// population order, x fastest, within the LDS space of its streaming phase
+// (the phase assignment itself is taken from halo.h: D3Q27 streams in two
+// phases that reuse one space, D3Q19 in one); when BX is 8 or 16, a
+// population moving along x only keeps just its entering x plane (extent 1
+// along x). A cell is identified by (phase, index).
 // Incoming: for each node n and population i whose source n - c_i lies
 // outside the tile along a set A of axes, the LDS read cell n + [c_i < 0]
 // must receive, exactly once, the ghost value of face 2*win + [c_win > 0]
@@ -37,6 +40,8 @@ static int failures = 0;
         }                                    \
     } while (0)
 
+constexpr int PHASE_KEY = 1 << 20;  // cell key = phase * PHASE_KEY + index within the phase
+
 // LDS cell of population i at array coordinates e (the geometry above)
 template <int Q, int BX, int BY, int BZ>
 int pop_cell(int i, const int e[3]) {
@@ -47,10 +52,12 @@ int pop_cell(int i, const int e[3]) {
         if (a == 0 && xonly) return 1;
         return B[a] + (c[a] != 0 ? 1 : 0);
     };
+    using P = PopLds<Q, BX, BY, BZ>;
     int off = 0;
-    for (int j = 1; j < i; j++) off += ext(j, 0) * ext(j, 1) * ext(j, 2);
+    for (int j = 1; j < i; j++)
+        if (P::phase(j) == P::phase(i)) off += ext(j, 0) * ext(j, 1) * ext(j, 2);
     const int ex = ext(i, 0) == 1 ? 0 : e[0];
-    return off + ex + ext(i, 0) * (e[1] + ext(i, 1) * e[2]);
+    return PHASE_KEY * P::phase(i) + off + ex + ext(i, 0) * (e[1] + ext(i, 1) * e[2]);
 }
 
 template <int Q, int BX, int BY, int BZ>
@@ -105,7 +112,7 @@ void check_shape() {
             const bool valid = (nc >= 64 || lane < nc) && u >= ch.ulo && u <= ch.uhi && v >= ch.vlo && v <= ch.vhi;
             if (!valid) continue;
             CHECK(plane == u + Bu * v, "lane/cell mismatch");
-            const int cell = ch.lds0 + u * ch.su + r * ch.sv;
+            const int cell = PHASE_KEY * PopLds<Q, BX, BY, BZ>::phase(ch.pop) + ch.lds0 + u * ch.su + r * ch.sv;
             written[cell]++;
             auto it = expect.find(cell);
             CHECK(it != expect.end(), "Q%d %dx%dx%d pop %d face %d writes unexpected cell %d", Q, BX, BY, BZ, ch.pop,
@@ -159,6 +166,7 @@ int main() {
     SHAPE(16, 8, 4)
     SHAPE(32, 4, 4)
     SHAPE(64, 4, 2)
+    SHAPE(16, 8, 8)
     if (failures) {
         std::printf("%d failures\n", failures);
         return 1;
