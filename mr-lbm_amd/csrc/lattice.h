@@ -228,16 +228,14 @@ __device__ __forceinline__ void reconstruct(const Node& n, float* g) {
     reconstruct_t<Q>(t, re0, n.drho, g);
 }
 
-// Moments of the streamed populations at a bulk node
-// (src/mlbm.cu:464-509): rho, u = sum f c / rho, m_ab = sum f c_a c_b / rho - cs2 d_ab,
-// then scaled by 3, 4.5, 9. From deviations: drho = sum g, j = sum g c,
-// m_aa = (sum g c_a^2 - drho/3) / rho, m_ab = sum g c_a c_b / rho. Sums are
+// Sums of streamed deviations: r = sum g, j = sum g c, s_ab = sum g c_a c_b,
 // formed from the pair sums p = g_i + g_i' and differences d = g_i - g_i'.
-// su, sm: power-of-two scales of the outputs U and M (the storage scales
-// of a 16-bit format, whose rounding then needs no further multiply); the
-// density deviation is not scaled.
+struct Sums {
+    float r, jx, jy, jz, sxx, syy, szz, sxy, sxz, syz;
+};
+
 template <int Q>
-__device__ __forceinline__ void bulk_moments(const float* g, float rho0, Node& n, float su = 1.0f, float sm = 1.0f) {
+__device__ __forceinline__ void moment_sums(const float* g, Sums& S) {
     const float p12 = g[1] + g[2], d12 = g[1] - g[2];
     const float p34 = g[3] + g[4], d34 = g[3] - g[4];
     const float p56 = g[5] + g[6], d56 = g[5] - g[6];
@@ -272,21 +270,43 @@ __device__ __forceinline__ void bulk_moments(const float* g, float rho0, Node& n
         jy = jy + ((da + db) - (dc + dd));
         jz = jz + ((da - db) + (dc - dd));
     }
+    S.r = r;
+    S.jx = jx, S.jy = jy, S.jz = jz;
+    S.sxx = sxx, S.syy = syy, S.szz = szz;
+    S.sxy = sxy, S.sxz = sxz, S.syz = syz;
+}
+
+// Moments of the streamed populations at a bulk node
+// (src/mlbm.cu:464-509): rho, u = sum f c / rho, m_ab = sum f c_a c_b / rho - cs2 d_ab,
+// then scaled by 3, 4.5, 9. From deviations: drho = sum g, j = sum g c,
+// m_aa = (sum g c_a^2 - drho/3) / rho, m_ab = sum g c_a c_b / rho.
+// su, sm: power-of-two scales of the outputs U and M (the storage scales
+// of a 16-bit format, whose rounding then needs no further multiply); the
+// density deviation is not scaled.
+__device__ __forceinline__ void bulk_from_sums(const Sums& S, float rho0, Node& n, float su = 1.0f, float sm = 1.0f) {
+    const float r = S.r;
     n.drho = r;
     n.rho = rho0 + r;
     const float inv = __builtin_amdgcn_rcpf(n.rho);
     n.inv = inv;
     const float u3 = (3.0f * su) * inv, m45 = (4.5f * sm) * inv, m9 = (9.0f * sm) * inv;
     const float r3 = r * (1.0f / 3.0f);
-    n.ux = jx * u3;
-    n.uy = jy * u3;
-    n.uz = jz * u3;
-    n.mxx = (sxx - r3) * m45;
-    n.myy = (syy - r3) * m45;
-    n.mzz = (szz - r3) * m45;
-    n.mxy = sxy * m9;
-    n.mxz = sxz * m9;
-    n.myz = syz * m9;
+    n.ux = S.jx * u3;
+    n.uy = S.jy * u3;
+    n.uz = S.jz * u3;
+    n.mxx = (S.sxx - r3) * m45;
+    n.myy = (S.syy - r3) * m45;
+    n.mzz = (S.szz - r3) * m45;
+    n.mxy = S.sxy * m9;
+    n.mxz = S.sxz * m9;
+    n.myz = S.syz * m9;
+}
+
+template <int Q>
+__device__ __forceinline__ void bulk_moments(const float* g, float rho0, Node& n, float su = 1.0f, float sm = 1.0f) {
+    Sums S;
+    moment_sums<Q>(g, S);
+    bulk_from_sums(S, rho0, n, su, sm);
 }
 
 // Boundary-condition class record (device table, BC_STRIDE floats per class).
@@ -306,30 +326,17 @@ __device__ __forceinline__ void bulk_moments(const float* g, float rho0, Node& n
 // x = (rho, P) = rho_I A^-1 (1, m_I).
 constexpr int BC_STRIDE = 72;
 
-// Moment-based boundary condition at a node of class `c` from the streamed
-// deviations g (f_i = w_i RHO_0 + g_i). The solve is split around the rest
-// state: (1, m_I) = (1, c) + (0, db) with db = (T - c r) / rho_I, where
-// r = sum_K g and T = sum_K g H, so every term the FP32 arithmetic forms is
-// a small deviation. Outputs unscaled rho, u, m exactly as a
-// bc_definition.inc fragment does (src/mlbm.cu:456-461); the caller scales.
-template <int Q, typename P>
-__device__ __forceinline__ void bc_solve(P c, const float* g, float rho0, Node& n) {
-    const uint32_t known = __float_as_uint(c[0]);
-    float r = 0.f, sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
-#pragma unroll
-    for (int i = 0; i < Q; i++) {
-        const float f = ((known >> i) & 1u) ? g[i] : 0.0f;
-        r = r + f;
-        if (cxv(i) != 0) sxx = sxx + f;
-        if (cyv(i) != 0) syy = syy + f;
-        if (czv(i) != 0) szz = szz + f;
-        if (cxv(i) * cyv(i) > 0) sxy = sxy + f;
-        if (cxv(i) * cyv(i) < 0) sxy = sxy - f;
-        if (cxv(i) * czv(i) > 0) sxz = sxz + f;
-        if (cxv(i) * czv(i) < 0) sxz = sxz - f;
-        if (cyv(i) * czv(i) > 0) syz = syz + f;
-        if (cyv(i) * czv(i) < 0) syz = syz - f;
-    }
+// Moment-based boundary condition at a node of class `c`, from the sums of
+// its known streamed deviations (f_i = w_i RHO_0 + g_i; moment_sums over g
+// with the unknown populations zeroed, see known_only). The solve is split
+// around the rest state: (1, m_I) = (1, c) + (0, db) with
+// db = (T - c r) / rho_I, where r = sum_K g and T = sum_K g H, so every term
+// the FP32 arithmetic forms is a small deviation. Outputs unscaled rho, u, m
+// exactly as a bc_definition.inc fragment does (src/mlbm.cu:456-461); the
+// caller scales.
+template <typename P>
+__device__ __forceinline__ void bc_from_sums(P c, const Sums& S, float rho0, Node& n) {
+    const float r = S.r, sxx = S.sxx, syy = S.syy, szz = S.szz, sxy = S.sxy, sxz = S.sxz, syz = S.syz;
     const float rI = rho0 * c[53] + r;
     const float inv = __builtin_amdgcn_rcpf(rI);  // v_rcp_f32 (1 ulp): the BC moments' rounding level
     const float r3 = r * (1.0f / 3.0f);
@@ -363,6 +370,25 @@ __device__ __forceinline__ void bc_solve(P c, const float* g, float rho0, Node& 
     n.myy = (c[65] + dy[4]) * iy0;
     n.myz = (c[66] + dy[5]) * iy0;
     n.mzz = (c[67] + dy[6]) * iy0;
+}
+
+// Zero the populations a boundary node does not know (bit i of `known`
+// clear): one signed bit-field extract (0 or all ones) and one AND each.
+template <int Q>
+__device__ __forceinline__ void known_only(uint32_t known, float* g) {
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        const int keep = __builtin_amdgcn_sbfe((int)known, i, 1);
+        g[i] = __int_as_float(__float_as_int(g[i]) & keep);
+    }
+}
+
+template <int Q, typename P>
+__device__ __forceinline__ void bc_solve(P c, float* g, float rho0, Node& n) {
+    known_only<Q>(__float_as_uint(c[0]), g);
+    Sums S;
+    moment_sums<Q>(g, S);
+    bc_from_sums(c, S, rho0, n);
 }
 
 // Scale unscaled u, m to the stored convention (src/mlbm.cu:500-509),

@@ -129,14 +129,20 @@ constexpr int staged_terms() { return Q == 27 ? Recon::N : Recon::N - 1; }
 
 // LDS of one tile update: the population arrays (PopLds), reused for the
 // staged terms (term k of node base b at k * E + b), and the node classes
-// (boundary body only: the bulk body needs none, which keeps D3Q19 8x8x8 at
-// 40,832 bytes, four workgroups per CU).
+// the outgoing-ghost writes read (boundary body only). The classes are
+// written with the staged terms, after the populations are consumed, so
+// where the population arrays are larger than the staged terms by a byte
+// per node they live in that slack (D3Q19: 8x8x8 keeps 40,836 bytes, four
+// workgroups per CU, in both bodies); otherwise in an array of their own.
 template <int Q, int BX, int BY, int BZ, bool HAS_BC>
 struct TileLds {
+    static constexpr int T = BX * BY * BZ;
     static constexpr int NP = PopLds<Q, BX, BY, BZ>::total();
     static constexpr int NS = staged_terms<Q>() * HaloGeom<BX, BY, BZ>::E;
+    static constexpr bool CLS_SLACK = HAS_BC && (NP - NS) * 4 >= T;
     float s[NP > NS ? NP : NS];
-    uint8_t cls[HAS_BC ? BX * BY * BZ : 4];
+    uint8_t cls_own[(HAS_BC && !CLS_SLACK) ? T : 4];
+    __device__ __forceinline__ uint8_t* cls() { return CLS_SLACK ? (uint8_t*)(s + NS) : cls_own; }
 };
 
 // HBM reads of one tile update (class bytes, stored moments, this wave's
@@ -232,7 +238,7 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* gout, uint
     constexpr int NIN = TileIn<Q, BX, BY, BZ, S>::NIN;
     constexpr int NOUT = (int)ChunkTables<Q, BX, BY, BZ>::out.size();
     float* s = L.s;
-    uint8_t* s_cls = L.cls;
+    uint8_t* s_cls = L.cls();
 
     const int t = thread_index<T>();
     const int tx = t % BX, ty = (t / BX) % BY, tz = t / (BX * BY);
@@ -242,7 +248,6 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* gout, uint
     const int base = tx + G::DX * (ty + G::DY * tz);
 
     const uint32_t cls = in.cls;
-    if constexpr (HAS_BC) s_cls[t] = (uint8_t)cls;
     const bool solid = HAS_BC && (cls == 255u);
 
     // FP16: the reconstruction reads the fetched binary16 words directly
@@ -328,9 +333,24 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* gout, uint
     });
 
     Node m;
-    if (HAS_BC && cls != 0u && cls != 255u) {
-        bc_solve<Q>(gptr(a.bc) + (size_t)cls * BC_STRIDE, g, a.rho0, m);
-        scale_moments(m, SU, SM);
+    if constexpr (HAS_BC) {
+        // boundary nodes: one pass of moment sums shared with the bulk nodes
+        // of the wave, over their known populations only (the others zeroed)
+        const bool bcn = cls != 0u && cls != 255u;
+        const uint64_t bcw = __builtin_amdgcn_ballot_w64(bcn);
+        if (bcw != 0) {  // uniform: waves without boundary nodes skip
+            uint32_t known = ~0u;
+            if (bcn) known = __float_as_uint(gptr(a.bc)[(size_t)cls * BC_STRIDE]);
+            known_only<Q>(known, g);
+        }
+        Sums sums;
+        moment_sums<Q>(g, sums);
+        if (bcn) {
+            bc_from_sums(gptr(a.bc) + (size_t)cls * BC_STRIDE, sums, a.rho0, m);
+            scale_moments(m, SU, SM);
+        } else {
+            bulk_from_sums(sums, a.rho0, m, SU, SM);
+        }
     } else {
         bulk_moments<Q>(g, a.rho0, m, SU, SM);
     }
@@ -366,12 +386,14 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* gout, uint
         constexpr int k = decltype(KK)::value;
         s[k * E + base] = rt.at(k);
     });
+    if constexpr (HAS_BC) s_cls[t] = (uint8_t)cls;
     __syncthreads();
     dispatch<NW>(w, [&](auto WW) {
         constexpr int W = decltype(WW)::value;
         static_for<(NOUT + NW - 1) / NW>([&](auto RR) {
             constexpr int K = W + NW * decltype(RR)::value;
-            if constexpr (K < NOUT) halo_store<Q, BX, BY, BZ, HAS_BC, K>(a, gout, lane, tile, bx, by, bz, s, s_cls);
+            if constexpr (K < NOUT)
+                halo_store<Q, BX, BY, BZ, HAS_BC, K>(a, gout, lane, tile, bx, by, bz, s, s_cls);
         });
     });
 }
@@ -390,13 +412,21 @@ constexpr int lds_waves_per_simd() {
     return w > 8 ? 8 : (w < 1 ? 1 : w);
 }
 
-// Register budget of a step body: the bulk body, and the FP32 boundary body
-// with x rows of at most 16 nodes (it fits the 80 VGPRs of 6 waves per SIMD
-// without spilling), take the waves the LDS allows; other boundary bodies
-// keep the compiler's choice (a tighter budget spills VGPRs to memory).
-template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S>
+// Register budget of a step body: the bulk body, the FP32 boundary bodies
+// with x rows of at most 16 nodes and the 16-bit D3Q19 boundary bodies with
+// 8-node rows (D3Q19: ~64 VGPRs at 8 waves per SIMD) take the waves the LDS
+// allows; other boundary bodies keep the compiler's choice (a tighter budget
+// spills VGPRs to memory). The ranged batched kernel also holds the next
+// item's fetch: its boundary bodies keep the compiler's choice and its
+// D3Q27 bulk bodies are held to 6 waves (80 VGPRs).
+template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S, bool RANGED = false>
 constexpr int step_body_waves() {
-    return (!HAS_BC || (S == 0 && BX <= 16)) ? lds_waves_per_simd<Q, BX, BY, BZ, HAS_BC>() : 1;
+    constexpr int w = lds_waves_per_simd<Q, BX, BY, BZ, HAS_BC>();
+    if constexpr (HAS_BC) {
+        if constexpr (RANGED || !(BX <= 16 && (S == 0 || (Q == 19 && BX <= 8)))) return 1;
+        return w;
+    }
+    return (RANGED && Q == 27 && w > 6) ? 6 : w;
 }
 
 // Tile step: block b updates tile list[b] (tile b when `list` is null), one
@@ -421,7 +451,7 @@ void k_step_tile(StepArgs a, const uint32_t* __restrict__ list) {
 // workgroups per CU); HAS_BC = true runs the tiles that hold boundary or
 // solid nodes, and tiles of lattices without a class map read class 0.
 template <int Q, int BX, int BY, int BZ, int S, bool HAS_BC>
-__global__ __launch_bounds__(BX* BY* BZ) __attribute__((amdgpu_waves_per_eu(step_body_waves<Q, BX, BY, BZ, HAS_BC, S>())))
+__global__ __launch_bounds__(BX* BY* BZ) __attribute__((amdgpu_waves_per_eu(step_body_waves<Q, BX, BY, BZ, HAS_BC, S, true>())))
 void k_step_batch(const InstArgs* __restrict__ tab,
                                                            const uint32_t* __restrict__ work, int nwork, int k) {
     __shared__ TileLds<Q, BX, BY, BZ, HAS_BC> L;
