@@ -274,7 +274,9 @@ int mrlbm_batch_set_grid(mrlbm_batch* b, int max_blocks);
  * graph_steps = g > 0: a hipGraph of g steps is captured (once per
  * membership) and replayed nsteps/g times (nsteps must be a multiple). */
 int mrlbm_batch_step(mrlbm_batch* b, int nsteps, int graph_steps);
-/* Same, bracketed by HIP events on the batch stream; *elapsed_ms. */
+/* Same, bracketed by HIP events on the batch stream; *elapsed_ms. A graph
+ * that does not exist yet is captured (not run) before the timed region;
+ * nsteps not a multiple of graph_steps is refused before anything runs. */
 int mrlbm_batch_time_steps(mrlbm_batch* b, int nsteps, int graph_steps, double* elapsed_ms);
 int mrlbm_batch_synchronize(mrlbm_batch* b);
 int mrlbm_batch_size(mrlbm_batch* b, int* members, int64_t* nodes);

@@ -134,7 +134,34 @@ void advance_members(mrlbm_batch* b, int nsteps) {
     }
 }
 
+// Graph of `per` steps for the current membership (captured, not run: the
+// members do not advance). Tables must be current (refresh).
+int ensure_graph(mrlbm_batch* b, int per) {
+    if (b->graph_valid && b->graph_steps == per) return MRLBM_OK;
+    drop_graph(b);
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_steps(b, per);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : fail(MRLBM_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    b->graph = g;
+    const hipError_t ie = hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        b->gexec = nullptr;
+        drop_graph(b);
+        return fail(MRLBM_EHIP, "graph instantiation failed: %s", hipGetErrorString(ie));
+    }
+    b->graph_steps = per;
+    b->graph_valid = true;
+    return MRLBM_OK;
+}
+
 int do_batch_steps(mrlbm_batch* b, int nsteps, int use_graph) {
+    if (use_graph && nsteps % use_graph)
+        return fail(MRLBM_EINVAL, "%d steps is not a multiple of the graph length %d", nsteps, use_graph);
     int rc = refresh(b);
     if (rc) return rc;
     if (b->members.empty() || nsteps == 0) return MRLBM_OK;
@@ -146,20 +173,7 @@ int do_batch_steps(mrlbm_batch* b, int nsteps, int use_graph) {
     }
     // graph of use_graph steps (captured once per membership), replayed
     const int per = use_graph;
-    if (nsteps % per) return fail(MRLBM_EINVAL, "%d steps is not a multiple of the graph length %d", nsteps, per);
-    if (!b->graph_valid || b->graph_steps != per) {
-        drop_graph(b);
-        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_steps(b, per);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(b->stream, &g);
-        if (rc) return rc;
-        if (e != hipSuccess) return fail(MRLBM_EHIP, "graph capture failed: %s", hipGetErrorString(e));
-        b->graph = g;
-        HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
-        b->graph_steps = per;
-        b->graph_valid = true;
-    }
+    if ((rc = ensure_graph(b, per))) return rc;
     for (int r = 0; r < nsteps / per; r++) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
     advance_members(b, nsteps);
     return MRLBM_OK;
@@ -241,13 +255,12 @@ int mrlbm_batch_step(mrlbm_batch* b, int nsteps, int graph_steps) {
 int mrlbm_batch_time_steps(mrlbm_batch* b, int nsteps, int graph_steps, double* ms) {
     if (!b || !ms || nsteps < 0) return fail(MRLBM_EINVAL, "bad argument");
     HIPCHK(hipSetDevice(b->device));
+    if (graph_steps > 0 && nsteps % graph_steps)
+        return fail(MRLBM_EINVAL, "%d steps is not a multiple of the graph length %d", nsteps, graph_steps);
     int rc = refresh(b);
     if (rc) return rc;
-    if (graph_steps > 0 && !b->graph_valid) {
-        // capture outside the timed region
-        rc = do_batch_steps(b, graph_steps, graph_steps);
-        if (rc) return rc;
-    }
+    // the graph is captured (not run) outside the timed region
+    if (graph_steps > 0 && !b->members.empty() && nsteps > 0 && (rc = ensure_graph(b, graph_steps))) return rc;
     HIPCHK(hipEventRecord(b->ev0, b->stream));
     rc = do_batch_steps(b, nsteps, graph_steps);
     if (rc) return rc;

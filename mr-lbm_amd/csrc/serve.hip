@@ -13,9 +13,8 @@
 // at most `quantum` steps (hipGraph-captured when a full quantum fits); a
 // round never passes a member's step budget, so members leave exactly at
 // their budget. A finished request leaves the batch at the round boundary;
-// its FP64 mass (and optionally its moments) are read during the next round
-// and its memory is released at the boundary after that. Latency is arrival
-// to results-on-host.
+// its FP64 mass (and optionally its moments) are read and its memory is
+// released during the next round. Latency is arrival to results-on-host.
 // Failures are per request: a request that cannot be served (extents not a
 // multiple of the tile, more than 8 spheres, no steps, larger than the whole
 // budget, an admission that fails) or whose lattice turns non-finite (an
@@ -177,6 +176,14 @@ int serve_loop(Dev& dev, const mrlbm_serve_params* sp, mrlbm_request* reqs, int 
     while (done < nreq) {
         if ((rc = admit())) return fail_all(rc);
         if ((rc = finish())) return fail_all(rc);
+        // memory of finished requests is released while the round runs (the
+        // frees cost ~1 ms of host time per lattice, which at the round
+        // boundary would leave the device idle)
+        for (auto& a : leaving) {
+            dev.release(a);
+            used -= a.bytes;
+        }
+        leaving.clear();
         if (in_flight) {
             double b = 0;
             if ((rc = dev.wait_round(&b, bad))) return fail_all(rc);
@@ -200,12 +207,6 @@ int serve_loop(Dev& dev, const mrlbm_serve_params* sp, mrlbm_request* reqs, int 
             // (the first pass ran as the round started)
             if ((rc = admit())) return fail_all(rc);
         }
-        // memory of finished requests is released while the device is idle
-        for (auto& a : leaving) {
-            dev.release(a);
-            used -= a.bytes;
-        }
-        leaving.clear();
         for (size_t i = 0; i < active.size();) {
             if (active[i].remaining > 0) {
                 i++;
