@@ -195,6 +195,42 @@ def serving_ducts(n_req=2000, rate=20.0, seed=5, max_batch=8):
     }
 
 
+def duct_batch(k=8, storage=1, steps=200, windows=3, seed=5):
+    """The served workload's kernels without the loop: the first k duct
+    requests of the seeded stream (FP16 storage as served, 8x8x8 tiles) in
+    one batch, stepped with hipGraph replays; median of `windows` HIP-event
+    windows of `steps` steps. The serving capacity of this mix."""
+    import mrlbm
+    from mrlbm import serve
+
+    reqs = list(serve.duct_requests(k, seed=seed))
+    lats = []
+    for r in reqs:
+        lat = mrlbm.Lattice((r.nx, r.ny, r.nz), q=19, tau=0.5 + 3.0 * r.nu, walls=(0, 1, 1), force=(r.gx, 0, 0),
+                            tile=(8, 8, 8), storage=storage, seed=r.seed)
+        sp = [r.sphere[i] for i in range(r.nspheres)]
+        lat.set_geometry(centres=[(s[0], s[1], s[2]) for s in sp], radii=[s[3] for s in sp])
+        lat.init_uniform()
+        lats.append(lat)
+    nodes = sum(r.nx * r.ny * r.nz for r in reqs)
+    b = mrlbm.Batch(q=19, tile=(8, 8, 8), storage=storage)
+    for lat in lats:
+        b.add(lat)
+    b.step(50, graph_steps=50)
+    per = sorted(b.time_steps(steps, graph_steps=50) / steps for _ in range(windows))
+    b.close()
+    for lat in lats:
+        lat.close()
+    return {
+        "workload": f"first {k} requests of the served duct stream (seed {seed}) in one batch, "
+                    f"{['FP32', 'FP16', 'BF16'][storage]} moments, 8x8x8 tiles, graphs of 50 steps",
+        "nodes": nodes,
+        "value": round(nodes / (per[len(per) // 2] * 1e-3) / 1e6, 1),
+        "unit": "MLUPS",
+        "step_ms_windows": [round(t, 4) for t in per],
+    }
+
+
 def launch_windows(lat, launches, windows=3):
     """Mean launch time of `lat`'s step in seconds: the median over `windows`
     windows of `launches` back-to-back launches, each timed with HIP events
@@ -313,6 +349,10 @@ def main():
                     help="comma-separated storage formats of the D3Q27 duct lines (FP32 first is the reference)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-cavity batching comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served duct stream")
+    ap.add_argument("--duct-batch", type=int, default=8,
+                    help="batch the first K served duct requests and time the kernels alone (0: skip)")
+    ap.add_argument("--duct-batch-only", action="store_true",
+                    help="print only the duct-batch timing (for profiling the served mix)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -321,6 +361,10 @@ def main():
         sys.exit(2)
 
     import mrlbm
+
+    if args.duct_batch_only:
+        print(json.dumps({"duct_batch": duct_batch(max(1, args.duct_batch))}), flush=True)
+        return
 
     n = args.n
     tile = tuple(int(v) for v in args.tile.split(",")) if args.tile else None
@@ -411,6 +455,9 @@ def main():
     if not args.no_serve:
         phase("served duct stream")
         out["serving"] = serving_ducts()
+    if args.duct_batch > 0:
+        phase("served duct mix, kernels alone")
+        out["duct_batch"] = duct_batch(args.duct_batch)
     if not args.no_variants:
         phase("FP16/BF16 storage")
         out["variants"] = reduced_precision_variants(n, tau, u0, max(200, args.steps), max(20, args.warmup))
