@@ -434,11 +434,17 @@ int mrlbm_init_taylor_green(mrlbm_ctx* c, double u0, int mode) {
 
 static int do_steps(mrlbm_ctx* c, int nsteps) {
     const bool split = c->p.kernel == MRLBM_KERNEL_TILE && c->has_bc && c->d_tiles;
+    const bool mixed = split && mixed_launch_supported(c->p.q, c->bx, c->by, c->bz, c->p.storage);
     for (int s = 0; s < nsteps; s++) {
         StepArgs a = c->args();
         hipError_t e;
         if (c->p.kernel == MRLBM_KERNEL_COLUMN) {
             e = launch_step_column(c->p.q, c->bx, c->by, c->bz, c->has_bc, c->p.storage, a, c->ntiles, c->stream);
+        } else if (split && mixed) {
+            // one launch: the all-bulk tiles with the light body, the rest
+            // with the boundary body
+            e = launch_step_tile_mixed(c->p.q, c->bx, c->by, c->bz, c->p.storage, a, c->d_tiles, c->ntiles,
+                                       c->n_bulk_tiles, c->stream);
         } else if (split) {
             // all-bulk tiles with the light body, the rest with boundaries;
             // both read the same ghost set and write disjoint data

@@ -108,17 +108,26 @@ int refresh(mrlbm_batch* b) {
 }
 
 int enqueue_steps(mrlbm_batch* b, int nsteps) {
-    // one launch over the all-bulk tiles, one over the boundary tiles; the
-    // two read the same ghost set and write disjoint data
+    // one launch over the all-bulk tiles, one over the boundary tiles (both
+    // in one launch where the two bodies run at the same occupancy and there
+    // is no grid limit); they read the same ghost set and write disjoint data
     const int nb = b->nbulk, nc = b->nwork - b->nbulk;
     const int gb = b->grid_limit > 0 ? std::min(b->grid_limit, nb) : nb;
     const int gc = b->grid_limit > 0 ? std::min(b->grid_limit, nc) : nc;
+    const bool mixed = b->grid_limit <= 0 && nb > 0 && nc > 0 &&
+                       mixed_launch_supported(b->q, b->bx, b->by, b->bz, b->storage);
     for (int k = 0; k < nsteps; k++) {
-        hipError_t e = launch_step_batch(b->q, b->bx, b->by, b->bz, b->storage, false, b->d_tab, b->d_work, nb, gb, k,
-                                         b->stream);
-        if (e == hipSuccess)
-            e = launch_step_batch(b->q, b->bx, b->by, b->bz, b->storage, true, b->d_tab, b->d_work + nb, nc, gc, k,
+        hipError_t e;
+        if (mixed) {
+            e = launch_step_batch_mixed(b->q, b->bx, b->by, b->bz, b->storage, b->d_tab, b->d_work, b->nwork, nb, k,
+                                        b->stream);
+        } else {
+            e = launch_step_batch(b->q, b->bx, b->by, b->bz, b->storage, false, b->d_tab, b->d_work, nb, gb, k,
                                   b->stream);
+            if (e == hipSuccess)
+                e = launch_step_batch(b->q, b->bx, b->by, b->bz, b->storage, true, b->d_tab, b->d_work + nb, nc, gc, k,
+                                      b->stream);
+        }
         if (e != hipSuccess) return fail(MRLBM_EHIP, "batched step launch failed: %s", hipGetErrorString(e));
     }
     hipError_t e = launch_batch_advance(b->d_tab, (int)b->members.size(), nsteps, b->stream);

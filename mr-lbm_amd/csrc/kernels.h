@@ -181,6 +181,14 @@ hipError_t launch_batch_advance(InstArgs* tab, int ninst, int nsteps, hipStream_
 hipError_t launch_step_tile(int q, int bx, int by, int bz, bool has_bc, int storage, const StepArgs& a,
                             const uint32_t* list, int n, hipStream_t s);
 bool tile_shape_supported(int q, int bx, int by, int bz, int storage);
+// Both bodies in one launch: the first nbulk of the n tiles (work items) are
+// all-bulk, the rest hold boundary or solid nodes. Only where the two bodies
+// run at the same occupancy (mixed_launch_supported).
+bool mixed_launch_supported(int q, int bx, int by, int bz, int storage);
+hipError_t launch_step_tile_mixed(int q, int bx, int by, int bz, int storage, const StepArgs& a, const uint32_t* list,
+                                  int n, int nbulk, hipStream_t s);
+hipError_t launch_step_batch_mixed(int q, int bx, int by, int bz, int storage, const InstArgs* tab,
+                                   const uint32_t* work, int nwork, int nbulk, int k, hipStream_t s);
 
 // Column kernel (step_column.hip): one (x,y) column per thread over a
 // z-segment of bz planes; bx*by = 256.

@@ -41,6 +41,37 @@ hipError_t launch_step_tile(int q, int bx, int by, int bz, bool has_bc, int stor
     }
 }
 
+bool mixed_launch_supported(int q, int bx, int by, int bz, int storage) {
+    switch (storage) {
+        case 0: return mixed_launch_s<0>(q, bx, by, bz);
+        case 1: return mixed_launch_s<1>(q, bx, by, bz);
+        case 2: return mixed_launch_s<2>(q, bx, by, bz);
+        default: return false;
+    }
+}
+
+hipError_t launch_step_tile_mixed(int q, int bx, int by, int bz, int storage, const StepArgs& a, const uint32_t* list,
+                                  int n, int nbulk, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    switch (storage) {
+        case 0: return launch_step_mixed_s<0>(q, bx, by, bz, &a, nullptr, list, n, nbulk, 0, s);
+        case 1: return launch_step_mixed_s<1>(q, bx, by, bz, &a, nullptr, list, n, nbulk, 0, s);
+        case 2: return launch_step_mixed_s<2>(q, bx, by, bz, &a, nullptr, list, n, nbulk, 0, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_step_batch_mixed(int q, int bx, int by, int bz, int storage, const InstArgs* tab,
+                                   const uint32_t* work, int nwork, int nbulk, int k, hipStream_t s) {
+    if (nwork <= 0) return hipSuccess;
+    switch (storage) {
+        case 0: return launch_step_mixed_s<0>(q, bx, by, bz, nullptr, tab, work, nwork, nbulk, k, s);
+        case 1: return launch_step_mixed_s<1>(q, bx, by, bz, nullptr, tab, work, nwork, nbulk, k, s);
+        case 2: return launch_step_mixed_s<2>(q, bx, by, bz, nullptr, tab, work, nwork, nbulk, k, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_init_ghosts(int q, int bx, int by, int bz, int storage, const StepArgs& a, float* const g[6],
                               int ntiles, hipStream_t s) {
     GhostOut go;

@@ -227,10 +227,9 @@ __device__ __forceinline__ void tile_fetch(const StepArgs& a, const float* gin, 
 // k_step_tile and of the batched kernel). The whole workgroup calls it
 // with the same tile. `ahead` is called once the inputs are consumed
 // (before the first barrier): it issues the next tile's fetch.
-template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S, typename Ahead>
-__device__ __forceinline__ void tile_update(const StepArgs& a, float* gout, uint32_t step, int tile,
-                                            TileLds<Q, BX, BY, BZ, HAS_BC>& L, const TileIn<Q, BX, BY, BZ, S>& in,
-                                            Ahead&& ahead) {
+template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S, typename Lds, typename Ahead>
+__device__ __forceinline__ void tile_update(const StepArgs& a, float* gout, uint32_t step, int tile, Lds& L,
+                                            const TileIn<Q, BX, BY, BZ, S>& in, Ahead&& ahead) {
     constexpr int T = BX * BY * BZ;
     constexpr int NW = T / 64;
     using G = HaloGeom<BX, BY, BZ>;
@@ -511,6 +510,52 @@ void k_step_batch1(const InstArgs* __restrict__ tab, const uint32_t* __restrict_
     tile_update<Q, BX, BY, BZ, HAS_BC, S>(ia.a, ia.g[1 - p], ia.step0 + (uint32_t)k, tile, L, in, []() {});
 }
 
+// Both bodies in one launch, where they run at the same occupancy (the same
+// LDS size and register budget: D3Q19 8x8x8 in every storage format, FP32
+// 16x8x8): blocks [0, nbulk) take all-bulk tiles with the bulk body, the
+// others tiles with boundary or solid nodes. One launch per step instead of
+// two: the second launch's ramp-up and the first one's tail overlap.
+template <int Q, int BX, int BY, int BZ, int S>
+constexpr bool mixed_launch() {
+    return step_body_waves<Q, BX, BY, BZ, false, S>() == step_body_waves<Q, BX, BY, BZ, true, S>() &&
+           sizeof(TileLds<Q, BX, BY, BZ, false>) == sizeof(TileLds<Q, BX, BY, BZ, true>);
+}
+
+template <int Q, int BX, int BY, int BZ, int S>
+__global__ __launch_bounds__(BX* BY* BZ) __attribute__((amdgpu_waves_per_eu(step_body_waves<Q, BX, BY, BZ, true, S>())))
+void k_step_tile_mixed(StepArgs a, const uint32_t* __restrict__ list, int nbulk) {
+    __shared__ TileLds<Q, BX, BY, BZ, true> L;
+    const int tile = (int)__builtin_amdgcn_readfirstlane(list[blockIdx.x]);
+    if ((int)blockIdx.x < nbulk) {
+        TileIn<Q, BX, BY, BZ, S> in;
+        tile_fetch<Q, BX, BY, BZ, false, S>(a, a.gin, tile, in);
+        tile_update<Q, BX, BY, BZ, false, S>(a, a.gout, a.step, tile, L, in, []() {});
+    } else {
+        TileIn<Q, BX, BY, BZ, S> in;
+        tile_fetch<Q, BX, BY, BZ, true, S>(a, a.gin, tile, in);
+        tile_update<Q, BX, BY, BZ, true, S>(a, a.gout, a.step, tile, L, in, []() {});
+    }
+}
+
+template <int Q, int BX, int BY, int BZ, int S>
+__global__ __launch_bounds__(BX* BY* BZ) __attribute__((amdgpu_waves_per_eu(step_body_waves<Q, BX, BY, BZ, true, S>())))
+void k_step_batch1_mixed(const InstArgs* __restrict__ tab, const uint32_t* __restrict__ work, int nbulk, int k) {
+    __shared__ TileLds<Q, BX, BY, BZ, true> L;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(work[blockIdx.x]);
+    const int inst = (int)(wv >> 20), tile = (int)(wv & 0xFFFFFu);
+    const InstArgs& ia = tab[inst];  // read in place (scalar loads)
+    const int p = (ia.cur0 + k) & 1;
+    if ((int)blockIdx.x < nbulk) {
+        TileIn<Q, BX, BY, BZ, S> in;
+        tile_fetch<Q, BX, BY, BZ, false, S>(ia.a, ia.g[p], tile, in);
+        tile_update<Q, BX, BY, BZ, false, S>(ia.a, ia.g[1 - p], ia.step0 + (uint32_t)k, tile, L, in, []() {});
+    } else {
+        TileIn<Q, BX, BY, BZ, S> in;
+        tile_fetch<Q, BX, BY, BZ, true, S>(ia.a, ia.g[p], tile, in);
+        tile_update<Q, BX, BY, BZ, true, S>(ia.a, ia.g[1 - p], ia.step0 + (uint32_t)k, tile, L, in, []() {});
+    }
+}
+
 // Initial ghost planes: every tile-face node writes its outgoing
 // populations, reconstructed from the stored moments, on every face.
 template <int Q, int BX, int BY, int BZ, int S>
@@ -590,6 +635,43 @@ hipError_t launch_step_batch_s(int q, int bx, int by, int bz, bool has_bc, const
             return has_bc ? launch_batch_qb<27, A, B, C, S, true>(tab, work, nwork, grid, k, s)           \
                           : launch_batch_qb<27, A, B, C, S, false>(tab, work, nwork, grid, k, s);         \
     }
+    MRLBM_TILE_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+template <int S>
+bool mixed_launch_s(int q, int bx, int by, int bz) {
+#define X(A, B, C)                                                                                        \
+    if (bx == A && by == B && bz == C) return q == 19 ? mixed_launch<19, A, B, C, S>() : mixed_launch<27, A, B, C, S>();
+    MRLBM_TILE_SHAPES(X)
+#undef X
+    return false;
+}
+
+template <int Q, int BX, int BY, int BZ, int S>
+static hipError_t launch_mixed_q(const StepArgs* a, const InstArgs* tab, const uint32_t* list, int n, int nbulk, int k,
+                                 hipStream_t s) {
+    if constexpr (mixed_launch<Q, BX, BY, BZ, S>()) {
+        if (a)
+            hipLaunchKernelGGL((k_step_tile_mixed<Q, BX, BY, BZ, S>), dim3(n), dim3(BX * BY * BZ), 0, s, *a, list, nbulk);
+        else
+            hipLaunchKernelGGL((k_step_batch1_mixed<Q, BX, BY, BZ, S>), dim3(n), dim3(BX * BY * BZ), 0, s, tab, list,
+                               nbulk, k);
+        return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+}
+
+// One launch over n tiles (a != null: list of one lattice's tiles) or work
+// items (a == null: the batch's work table), the first nbulk all-bulk.
+template <int S>
+hipError_t launch_step_mixed_s(int q, int bx, int by, int bz, const StepArgs* a, const InstArgs* tab,
+                               const uint32_t* list, int n, int nbulk, int k, hipStream_t s) {
+#define X(A, B, C)                                                                                        \
+    if (bx == A && by == B && bz == C)                                                                    \
+        return q == 19 ? launch_mixed_q<19, A, B, C, S>(a, tab, list, n, nbulk, k, s)                     \
+                       : launch_mixed_q<27, A, B, C, S>(a, tab, list, n, nbulk, k, s);
     MRLBM_TILE_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;

@@ -23,6 +23,11 @@ template <int S>
 hipError_t launch_step_batch_s(int q, int bx, int by, int bz, bool has_bc, const InstArgs* tab, const uint32_t* work,
                                int nwork, int grid, int k, hipStream_t s);
 template <int S>
+bool mixed_launch_s(int q, int bx, int by, int bz);
+template <int S>
+hipError_t launch_step_mixed_s(int q, int bx, int by, int bz, const StepArgs* a, const InstArgs* tab,
+                               const uint32_t* list, int n, int nbulk, int k, hipStream_t s);
+template <int S>
 hipError_t launch_init_ghosts_s(int q, int bx, int by, int bz, const StepArgs& a, const GhostOut& go, int ntiles,
                                 hipStream_t s);
 
