@@ -80,6 +80,8 @@ struct mrlbm_ctx {
         a.ntiles = ntiles;
         a.ntx = ntx;
         a.nty = nty;
+        a.by_ntx = mrlbm::make_fastdiv((uint32_t)ntx);
+        a.by_nty = mrlbm::make_fastdiv((uint32_t)nty);
         a.ntz = ntz;
         a.nx = p.nx;
         a.ny = p.ny;

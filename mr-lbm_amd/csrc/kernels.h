@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fastdiv.h"
 #include "lattice.h"
 
 namespace mrlbm {
@@ -105,6 +106,7 @@ struct StepArgs {
     float* gout;             // ghost set written
     int ntiles;
     int ntx, nty, ntz;       // tiles per axis
+    FastDiv by_ntx, by_nty;  // division by ntx, by nty (tile coordinates)
     int nx, ny, nz;
     int wall_x, wall_y, wall_z;
     float rho0;

@@ -50,7 +50,8 @@ __global__ __launch_bounds__(BX* BY) void k_step_column(StepArgs a) {
     const int t = threadIdx.x;
     const int tx = t % BX, ty = t / BX;
     const int tile = blockIdx.x;
-    const int bx = tile % a.ntx, by = (tile / a.ntx) % a.nty, bz = tile / (a.ntx * a.nty);
+    const uint32_t tq = a.by_ntx.div((uint32_t)tile), tz = a.by_nty.div(tq);
+    const int bx = (int)((uint32_t)tile - tq * (uint32_t)a.ntx), by = (int)(tq - tz * (uint32_t)a.nty), bz = (int)tz;
     const int x = bx * BX + tx, y = by * BY + ty;
     const size_t mbase = (size_t)tile * 10 * T + t;
     const size_t cbase = (size_t)tile * T + t;

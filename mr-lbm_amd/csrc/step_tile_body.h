@@ -169,12 +169,21 @@ __device__ __forceinline__ int thread_index() {
     return (int)(t & (uint32_t)(T - 1));  // T is a power of two: 0 <= t < T stays known
 }
 
+// Tile coordinates of a tile index (x fastest): scalar multiply-high
+// divisions (FastDiv), the tile index being uniform.
+__device__ __forceinline__ void tile_xyz(const StepArgs& a, int tile, int& bx, int& by, int& bz) {
+    const uint32_t t = (uint32_t)tile;
+    const uint32_t q = a.by_ntx.div(t);
+    const uint32_t z = a.by_nty.div(q);
+    bx = (int)(t - q * (uint32_t)a.ntx);
+    by = (int)(q - z * (uint32_t)a.nty);
+    bz = (int)z;
+}
+
 // Tile coordinates of a tile index and its periodic neighbours, as the
 // reference's bxm1/bxp1 (src/mlbm.cu:115-122).
 __device__ __forceinline__ void tile_coords(const StepArgs& a, int tile, int& bx, int& by, int& bz, Nbr& nb) {
-    bx = tile % a.ntx;
-    by = (tile / a.ntx) % a.nty;
-    bz = tile / (a.ntx * a.nty);
+    tile_xyz(a, tile, bx, by, bz);
     nb.ntx = a.ntx;
     nb.nty = a.nty;
     nb.ntiles = a.ntiles;
@@ -243,7 +252,8 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* gout, uint
     const int tx = t % BX, ty = (t / BX) % BY, tz = t / (BX * BY);
     const int lane = t % 64;
     const int w = __builtin_amdgcn_readfirstlane(t / 64);
-    const int bx = tile % a.ntx, by = (tile / a.ntx) % a.nty, bz = tile / (a.ntx * a.nty);
+    int bx, by, bz;
+    tile_xyz(a, tile, bx, by, bz);
     const int base = tx + G::DX * (ty + G::DY * tz);
 
     const uint32_t cls = in.cls;

@@ -158,6 +158,22 @@ def test_spheres_voxelisation():
     np.testing.assert_array_equal(s, (x - 8) ** 2 + (y - 8) ** 2 + (z - 8) ** 2 <= 9)
 
 
+def test_fastdiv_exact(tmp_path):
+    """The step kernels' tile coordinates use a multiply-high division by the
+    tile counts (fastdiv.h): exact against the hardware division for every
+    divisor 1..4096, large divisors and edge dividends (tests/fastdiv_check.cpp)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if not gxx:
+        pytest.skip("g++ not available")
+    exe = tmp_path / "fastdiv_check"
+    src = os.path.join(os.path.dirname(__file__), "fastdiv_check.cpp")
+    subprocess.run([gxx, "-std=c++17", "-O2", src, "-o", str(exe)], check=True, capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout
+
+
 def test_halo_chunk_tables(tmp_path):
     """The tile kernel's ghost-chunk tables move every halo cell once from
     the right ghost-plane cell and read only inside the owner's plane
