@@ -164,7 +164,7 @@ def batching_cavities(n_lat=64, steps=2000, seed=2024, check=4):
     }
 
 
-def serving_ducts(n_req=2000, rate=20.0, seed=5, max_batch=8):
+def serving_ducts(n_req=2000, rate=20.0, seed=5, max_batch=6):
     """The duct request stream (2000 requests, Poisson rate 20/s, FP16
     storage, 200 GB budget) served in real time with continuous batching.
     At most max_batch lattices share the GPU: the admitted ducts saturate it
