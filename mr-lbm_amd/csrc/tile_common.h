@@ -203,14 +203,15 @@ __device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t st
             const uint32_t r0 = sr_operand<k>(w0, w1, w2), r1 = sr_operand<k + 1>(w0, w1, w2);
             uint32_t b;
             if constexpr (S == 1) {
-                half2_t h = {};
+                // both halves are written: start from any register (no zeroing move)
+                half2_t h = __builtin_bit_cast(half2_t, r0);
                 const float x0 = k == 0 ? *v[k] * store_scale(1, 0) : *v[k];  // U, M: already scaled
                 h = __builtin_amdgcn_cvt_sr_f16_f32(h, x0, r0, false);
                 h = __builtin_amdgcn_cvt_sr_f16_f32(h, *v[k + 1], r1, true);
                 b = __builtin_bit_cast(uint32_t, h);
                 pk[k / 2] = b;
             } else {
-                bf162_t h = {};
+                bf162_t h = __builtin_bit_cast(bf162_t, r0);
                 h = __builtin_amdgcn_cvt_sr_bf16_f32(h, *v[k], r0, false);
                 h = __builtin_amdgcn_cvt_sr_bf16_f32(h, *v[k + 1], r1, true);
                 b = __builtin_bit_cast(uint32_t, h);
