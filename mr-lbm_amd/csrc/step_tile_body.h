@@ -408,7 +408,7 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* gout, uint
 }
 
 // Waves per SIMD the LDS of a workgroup allows (160 KB per CU, 4 SIMDs),
-// capped at 8. The register allocator is asked for that many waves, so that
+// capped at 8 (gfx950 holds at most 8 waves per SIMD). The register allocator is asked for that many waves, so that
 // a shape whose LDS admits four 512-thread workgroups per CU is not held
 // back by scalar registers (the step arguments alone take ~70 of them; a
 // few are kept in VGPR lanes instead).
@@ -426,9 +426,9 @@ constexpr int lds_waves_per_simd() {
 // bodies with 8-node rows (~64 VGPRs at 8 waves per SIMD) take the waves the
 // LDS allows; the FP32 D3Q27 boundary bodies with rows of at most 16 nodes
 // at most 6 waves (80 VGPRs); other boundary bodies keep the compiler's
-// choice (a tighter budget spills VGPRs to memory). The ranged batched kernel also holds the next
-// item's fetch: its boundary bodies and its D3Q27 bodies keep the
-// compiler's choice.
+// choice (a tighter budget spills VGPRs to memory). The ranged batched
+// kernel also holds the next item's fetch: its boundary bodies and its
+// D3Q27 bodies keep the compiler's choice.
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S, bool RANGED = false>
 constexpr int step_body_waves() {
     constexpr int w = lds_waves_per_simd<Q, BX, BY, BZ, HAS_BC>();
