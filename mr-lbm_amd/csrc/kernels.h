@@ -10,12 +10,13 @@
 
 namespace mrlbm {
 
-// Storage codecs. Arithmetic is always FP32; a codec maps a stored value
-// to/from float. Loads are non-temporal: moments are streamed once per step. Reduced-precision codecs store v * scale (a power of two,
-// so the scaling is exact) and round stochastically: `rnd` carries uniform
-// random bits (one draw per value, counter-based, see sr_bits below); adding
-// them below the kept mantissa bits and truncating gives an unbiased
-// rounding, so per-step changes smaller than half an ulp are not lost.
+// Storage codecs. Arithmetic is always FP32; a codec loads a stored value
+// and decodes it to float. Loads are non-temporal: moments are streamed
+// once per step. Reduced-precision formats hold v * scale (a power of two,
+// so the scaling is exact), rounded stochastically when the step kernels
+// store them (quantize in tile_common.h: the hardware stochastic
+// conversions with counter-based random operands, see sr_bits below), so
+// per-step changes smaller than half an ulp are not lost on average.
 // Storage scale of moment k (0 drho, 1-3 U, 4-9 M), a power of two:
 // binary16 keeps the expected magnitudes (|drho|, |M| up to ~0.1, |U| up to
 // ~0.3 for Mach < 0.2) near 2^8..2^9, far from both its overflow (65504) and
@@ -47,17 +48,11 @@ template <> struct Codec<0> {
     using T = float;
     template <typename P>
     __device__ __forceinline__ static float ld(P p, size_t i, float) { return __builtin_nontemporal_load(&p[i]); }
-    __device__ __forceinline__ static T enc(float v, float, uint32_t) { return v; }
     __device__ __forceinline__ static float dec(T b, float) { return b; }
 };
 template <> struct Codec<1> {  // IEEE binary16, 13 dropped bits rounded stochastically
     using T = uint16_t;
     __device__ __forceinline__ static float dec(uint32_t b, float rscale) { return f16_scaled<false>(b, rscale); }
-    __device__ __forceinline__ static T enc(float v, float scale, uint32_t rnd) {
-        const uint32_t u = __float_as_uint(v * scale);
-        const float t = __uint_as_float((u + (rnd & 0x1FFFu)) & ~0x1FFFu);
-        return __builtin_bit_cast(T, (_Float16)t);  // exact for normal binary16 values
-    }
     template <typename P>
     __device__ __forceinline__ static float ld(P p, size_t i, float rscale) { return dec(__builtin_nontemporal_load(&p[i]), rscale); }
 };
@@ -67,16 +62,13 @@ template <> struct Codec<1> {  // IEEE binary16, 13 dropped bits rounded stochas
 template <> struct Codec<2> {
     using T = uint16_t;
     __device__ __forceinline__ static float dec(uint32_t b, float) { return __uint_as_float(b << 16); }
-    __device__ __forceinline__ static T enc(float v, float, uint32_t rnd) {
-        const uint32_t u = __float_as_uint(v);
-        return (T)((u + (rnd & 0xFFFFu)) >> 16);
-    }
     template <typename P>
     __device__ __forceinline__ static float ld(P p, size_t i, float rscale) { return dec(__builtin_nontemporal_load(&p[i]), rscale); }
 };
 
-// Register type of a fetched stored moment: the 16-bit formats are held
-// zero-extended in 32-bit registers (the zero extension is part of the load).
+// Register type of a fetched stored moment: the 16-bit formats are held in
+// 32-bit registers (bfloat16 zero-extended by the load; binary16 with an
+// undefined high half, which the reconstruction does not read).
 template <int S> struct Fetched { using T = uint32_t; };
 template <> struct Fetched<0> { using T = float; };
 

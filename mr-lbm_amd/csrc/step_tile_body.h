@@ -214,8 +214,17 @@ __device__ __forceinline__ void tile_fetch(const StepArgs& a, const float* gin, 
     // moments are read and written exactly once per step: non-temporal
     // accesses keep them from displacing the ghost-plane lines that
     // neighbouring tiles read (measured +8 % on 256^3 FP32)
-    for (int k = 0; k < 10; k++)
-        in.m[k] = (typename Fetched<S>::T)__builtin_nontemporal_load(&mom[mb + (size_t)k * T]);
+    for (int k = 0; k < 10; k++) {
+        if constexpr (S == 1) {
+            // binary16: the reconstruction reads only the low half of the
+            // word, so the high half is left undefined (no zero extension)
+            typedef _Float16 __attribute__((ext_vector_type(1))) half1_t;
+            const half1_t x = {__builtin_nontemporal_load((const _Float16*)&mom[mb + (size_t)k * T])};
+            in.m[k] = __builtin_bit_cast(uint32_t, __builtin_shufflevector(x, x, 0, -1));
+        } else {
+            in.m[k] = (typename Fetched<S>::T)__builtin_nontemporal_load(&mom[mb + (size_t)k * T]);
+        }
+    }
     // wave W owns incoming chunks W, W + NW, ...; every wave fills all
     // RMAX slots (zeros past its last chunk) so that no slot index depends
     // on the wave (a wave-dependent index would move the array to scratch)

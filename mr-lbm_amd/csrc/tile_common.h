@@ -63,26 +63,32 @@ __device__ __forceinline__ void decode_node(const StepArgs& a, const typename Fe
 // fraction plus the random operand, read as a fraction of 2^32, reaches one
 // (measured on MI355X: for a value halfway between two neighbours only
 // operands with bit 31 set round up), so the operand's top bits decide and
-// each value needs its own top bits. Ten 32-bit operands with ten distinct
-// top bytes come from three words: a counter-based hash of (node, step,
-// seed) (sr_bits), two multiplicative successors, and funnel shifts of
-// word pairs; the same node, step and seed round alike in any launch shape
-// or batch. binary16 stores v times a power-of-two scale (exact) and the
-// moments are decoded back from the packed halves; bfloat16 has FP32's
-// exponent range and is stored unscaled.
+// each value needs its own top bits. Ten 32-bit operands come from three
+// words: a counter-based hash of (node, step, seed) (sr_bits) and two
+// 24-bit multiplicative successors of it (of bytes 0-2 and of bytes 1-3;
+// full-rate multiplies), each operand a funnel shift whose top byte is one
+// of w0.b3..b0, w1.b3..b1, w2.b3..b1. A successor's low byte depends on a
+// single byte of the hash, so it never leads an operand: any two operands'
+// top bits are close to independent (chi-square of their top four bits
+// ~1.4x its degrees of freedom over 2^20 nodes, where leading with the low
+// bytes of full multiplicative successors put three operands' top bytes on
+// one byte of the hash). The same node, step and seed round alike in any
+// launch shape or batch. binary16 stores v times a power-of-two scale
+// (exact) and the moments are decoded back from the packed halves;
+// bfloat16 has FP32's exponent range and is stored unscaled.
 template <int K>
 __device__ __forceinline__ uint32_t sr_operand(uint32_t w0, uint32_t w1, uint32_t w2) {
-    // top bytes: w0.b3, w0.b0, w0.b1, w0.b2, w1.b3, w1.b0, w2.b0, w2.b1, w2.b2, w2.b3
+    // top bytes: w0.b3, w0.b2, w0.b1, w0.b0, w1.b3, w1.b2, w1.b1, w2.b3, w2.b2, w2.b1
     if constexpr (K == 0) return w0;
-    else if constexpr (K == 1) return __builtin_amdgcn_alignbit(w0, w1, 8);
+    else if constexpr (K == 1) return __builtin_amdgcn_alignbit(w0, w1, 24);
     else if constexpr (K == 2) return __builtin_amdgcn_alignbit(w0, w1, 16);
-    else if constexpr (K == 3) return __builtin_amdgcn_alignbit(w0, w1, 24);
+    else if constexpr (K == 3) return __builtin_amdgcn_alignbit(w0, w1, 8);
     else if constexpr (K == 4) return w1;
-    else if constexpr (K == 5) return __builtin_amdgcn_alignbit(w1, w2, 8);
-    else if constexpr (K == 6) return __builtin_amdgcn_alignbit(w2, w0, 8);
-    else if constexpr (K == 7) return __builtin_amdgcn_alignbit(w2, w0, 16);
+    else if constexpr (K == 5) return __builtin_amdgcn_alignbit(w1, w2, 24);
+    else if constexpr (K == 6) return __builtin_amdgcn_alignbit(w1, w2, 16);
+    else if constexpr (K == 7) return w2;
     else if constexpr (K == 8) return __builtin_amdgcn_alignbit(w2, w0, 24);
-    else return w2;
+    else return __builtin_amdgcn_alignbit(w2, w0, 16);
 }
 
 // Reconstruction terms straight from binary16 moment words (storage format
@@ -196,7 +202,7 @@ __device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t st
         typedef _Float16 __attribute__((ext_vector_type(2))) half2_t;
         typedef __bf16 __attribute__((ext_vector_type(2))) bf162_t;
         const uint32_t w0 = sr_bits(node_id, step, a.seed);
-        const uint32_t w1 = w0 * 0x9E3779B1u, w2 = w0 * 0x85EBCA77u;
+        const uint32_t w1 = __umul24(w0, 0x9E3779u), w2 = __umul24(w0 >> 8, 0xEBCA77u);
         float* v[10] = {&n.drho, &n.ux, &n.uy, &n.uz, &n.mxx, &n.mxy, &n.mxz, &n.myy, &n.myz, &n.mzz};
         static_for<5>([&](auto PP) {
             constexpr int k = 2 * decltype(PP)::value;
