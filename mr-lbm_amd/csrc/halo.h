@@ -44,12 +44,15 @@ struct Chunk {
 };
 // Incoming chunks index the population arrays (PopLds); outgoing chunks
 // index the face node in the staged-term layout, node base
-// x + (BX+1) (y + (BY+1) z) of term array k at k * HaloGeom::E.
+// x + (BX+1) (y + BY z) of term array k at k * HaloGeom::E. The one
+// padding column makes the row stride odd: a wave's staged-term reads of
+// an x face (rows along y, 8 x 8 cells) and of a y face (rows along x, z
+// stride (BX+1) BY = 72 at 8x8x8) fall into 64 distinct LDS banks.
 
 template <int BX, int BY, int BZ>
 struct HaloGeom {
-    static constexpr int DX = BX + 1, DY = BY + 1, DZ = BZ + 1;
-    static constexpr int E = DX * DY * DZ;  // cells per population array
+    static constexpr int DX = BX + 1, DY = BY, DZ = BZ;
+    static constexpr int E = DX * DY * DZ;  // cells per staged-term array
     static constexpr int B[3] = {BX, BY, BZ};
     static constexpr int STRIDE[3] = {1, DX, DX * DY};
 };

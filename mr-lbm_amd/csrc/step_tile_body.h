@@ -433,18 +433,18 @@ constexpr int lds_waves_per_simd() {
 // Register budget of a step body: the bulk body, the FP32 D3Q19 boundary
 // bodies with x rows of at most 16 nodes and the 16-bit D3Q19 boundary
 // bodies with 8-node rows (~64 VGPRs at 8 waves per SIMD) take the waves the
-// LDS allows; the FP32 D3Q27 boundary bodies with rows of at most 16 nodes
-// at most 6 waves (80 VGPRs); other boundary bodies keep the compiler's
-// choice (a tighter budget spills VGPRs to memory). The ranged batched
-// kernel also holds the next item's fetch: its boundary bodies and its
-// D3Q27 bodies keep the compiler's choice.
+// LDS allows; the D3Q27 boundary bodies with rows of at most 16 nodes at most
+// 6 waves (80 VGPRs; at 8 waves they spill); other boundary bodies keep the
+// compiler's choice (a tighter budget spills VGPRs to memory). The ranged
+// batched kernel also holds the next item's fetch: its boundary bodies and
+// its D3Q27 bodies keep the compiler's choice.
 template <int Q, int BX, int BY, int BZ, bool HAS_BC, int S, bool RANGED = false>
 constexpr int step_body_waves() {
     constexpr int w = lds_waves_per_simd<Q, BX, BY, BZ, HAS_BC>();
     if constexpr (HAS_BC) {
         if constexpr (RANGED) return 1;
         if constexpr (Q == 19 && BX <= (S == 0 ? 16 : 8)) return w;
-        if constexpr (Q == 27 && S == 0 && BX <= 16) return w > 6 ? 6 : w;
+        if constexpr (Q == 27 && BX <= 16) return w > 6 ? 6 : w;
         return 1;
     }
     return (RANGED && Q == 27) ? 1 : w;

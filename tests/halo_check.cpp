@@ -16,7 +16,7 @@
 // coordinates; no other cell may be written. Every load index must lie in
 // the owner's plane of that slot.
 // Outgoing: every face cell of every slot is written exactly once, by a lane
-// whose LDS index is the face node's base x + (BX+1)(y + (BY+1) z) in the
+// whose LDS index is the face node's base x + (BX+1)(y + BY z) in the
 // staged-term layout.
 // Prints "ok" and exits 0 on success.
 #include <cstdio>
@@ -143,7 +143,9 @@ void check_shape() {
             node[a] = (ch.face & 1) ? B[a] - 1 : 0;
             node[ua] = u;
             node[va] = v;
-            const int nodebase = node[0] + (BX + 1) * (node[1] + (BY + 1) * node[2]);
+            // staged-term layout: one padding column in x, rows of BY, planes of BZ
+            const int nodebase = node[0] + (BX + 1) * (node[1] + BY * node[2]);
+            CHECK(nodebase < (HaloGeom<BX, BY, BZ>::E), "staged base outside its term array");
             CHECK(ch.lds0 + u * ch.su + r * ch.sv == nodebase, "outgoing LDS cell mismatch Q%d pop %d", Q, i);
             (void)c;
             out_written[std::make_tuple(ch.face, ch.slot, plane)]++;
