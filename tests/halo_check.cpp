@@ -17,7 +17,8 @@
 // the owner's plane of that slot.
 // Outgoing: every face cell of every slot is written exactly once, by a lane
 // whose LDS index is the face node's base x + (BX+1)(y + BY z) in the
-// staged-term layout.
+// staged-term layout; at 8x8x8 the 64 lanes of an x- or y-face chunk read
+// 64 distinct LDS banks (address mod 64).
 // Prints "ok" and exits 0 on success.
 #include <cstdio>
 #include <map>
@@ -149,6 +150,19 @@ void check_shape() {
             CHECK(ch.lds0 + u * ch.su + r * ch.sv == nodebase, "outgoing LDS cell mismatch Q%d pop %d", Q, i);
             (void)c;
             out_written[std::make_tuple(ch.face, ch.slot, plane)]++;
+        }
+    }
+    // 8x8x8: a wave's staged-term reads of an x or y face hit 64 distinct
+    // LDS banks (the odd row stride of the staged layout)
+    if (BX == 8 && BY == 8 && BZ == 8) {
+        for (const Chunk& ch : ChunkTables<Q, BX, BY, BZ>::out) {
+            if (ch.face >= 4) continue;
+            std::set<int> banks;
+            const int Bu = face_bu<BX, BY, BZ>(ch.face);
+            for (int lane = 0; lane < 64 && lane < Bu * ch.rows; lane++)
+                banks.insert((ch.lds0 + (lane % Bu) * ch.su + (lane / Bu) * ch.sv) % 64);
+            CHECK((int)banks.size() == (Bu * ch.rows < 64 ? Bu * ch.rows : 64),
+                  "Q%d face %d chunk: %d distinct banks", Q, ch.face, (int)banks.size());
         }
     }
     for (int f = 0; f < 6; f++) {
