@@ -29,17 +29,17 @@ __host__ __device__ constexpr float store_scale(int storage, int k) {
 
 // binary16 value (low or high half of `b`) times a power-of-two scale in one
 // mixed-precision FMA: v_fma_mix_f32 converts the half exactly and the
-// addend -0 makes the result x * s bit for bit, signed zeros included (the
-// same value as a conversion followed by a multiply, one instruction less).
-// The scale is uniform (an SGPR operand).
+// addend -0 (the inline constant 0 with the negate modifier) makes the
+// result x * s bit for bit, signed zeros included (the same value as a
+// conversion followed by a multiply, one instruction less). The scale is
+// uniform (an SGPR operand).
 template <bool HI>
 __device__ __forceinline__ float f16_scaled(uint32_t b, float s) {
     float r;
-    const float nz = -0.0f;
     if constexpr (HI)
-        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(b), "s"(s), "v"(nz));
+        asm("v_fma_mix_f32 %0, %1, %2, neg(0) op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(b), "s"(s));
     else
-        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(b), "s"(s), "v"(nz));
+        asm("v_fma_mix_f32 %0, %1, %2, neg(0) op_sel_hi:[1,0,0]" : "=v"(r) : "v"(b), "s"(s));
     return r;
 }
 

@@ -104,12 +104,22 @@ __device__ __forceinline__ uint32_t sr_operand(uint32_t w0, uint32_t w1, uint32_
 // of the two they use.
 template <bool HI>
 __device__ __forceinline__ float mix_mul(float a, uint32_t b) {  // a * b, one rounding (b: binary16 half)
+    // addend -0 (the inline constant 0 with the negate modifier): the result
+    // is the product itself, signed zero included
     float r;
-    const float nz = -0.0f;  // addend -0: the result is the product itself, signed zero included
     if constexpr (HI)
-        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[0,1,0] op_sel_hi:[0,1,0]" : "=v"(r) : "v"(a), "v"(b), "v"(nz));
+        asm("v_fma_mix_f32 %0, %1, %2, neg(0) op_sel:[0,1,0] op_sel_hi:[0,1,0]" : "=v"(r) : "v"(a), "v"(b));
     else
-        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,1,0]" : "=v"(r) : "v"(a), "v"(b), "v"(nz));
+        asm("v_fma_mix_f32 %0, %1, %2, neg(0) op_sel_hi:[0,1,0]" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+template <bool HI>
+__device__ __forceinline__ float mix_mul_k(float k, uint32_t b) {  // k * b with a uniform factor k (in an SGPR)
+    float r;
+    if constexpr (HI)
+        asm("v_fma_mix_f32 %0, %1, %2, neg(0) op_sel:[0,1,0] op_sel_hi:[0,1,0]" : "=v"(r) : "s"(k), "v"(b));
+    else
+        asm("v_fma_mix_f32 %0, %1, %2, neg(0) op_sel_hi:[0,1,0]" : "=v"(r) : "s"(k), "v"(b));
     return r;
 }
 template <bool HA, bool HB>
@@ -145,6 +155,9 @@ __device__ __forceinline__ void recon_terms_f16(const uint32_t* w, float rho0, f
     auto mul = [&](float a, auto K) {
         return mix_mul<PACKED && (decltype(K)::value & 1)>(a, W(K));
     };
+    auto mulk = [&](float k, auto K) {  // k: a compile-time constant
+        return mix_mul_k<PACKED && (decltype(K)::value & 1)>(k, W(K));
+    };
     using std::integral_constant;
     const integral_constant<int, 0> k0;
     const integral_constant<int, 1> k1;
@@ -156,7 +169,7 @@ __device__ __forceinline__ void recon_terms_f16(const uint32_t* w, float rho0, f
     const integral_constant<int, 7> k7;
     const integral_constant<int, 8> k8;
     const integral_constant<int, 9> k9;
-    drho = mul(1.0f / sd, k0);
+    drho = mulk(1.0f / sd, k0);
     rho = rho0 + drho;
     const float ru = rho * (1.0f / su), rm = rho * (1.0f / sm);
     // (mxx + myy) + mzz in the storage scale, then as recon_re0
@@ -172,9 +185,9 @@ __device__ __forceinline__ void recon_terms_f16(const uint32_t* w, float rho0, f
     t.rxy = mul(rm, k5);
     t.rxz = mul(rm, k6);
     t.ryz = mul(rm, k8);
-    t.d1 = mul(w1 / sd, k0);
-    t.d2 = mul(w2 / sd, k0);
-    if constexpr (Q == 27) t.d3 = mul((float)VS<Q>::wd(26) / sd, k0);
+    t.d1 = mulk(w1 / sd, k0);
+    t.d2 = mulk(w2 / sd, k0);
+    if constexpr (Q == 27) t.d3 = mulk((float)VS<Q>::wd(26) / sd, k0);
     else t.d3 = 0.0f;
 }
 
