@@ -38,6 +38,17 @@ __device__ __forceinline__ MRLBM_GLOBAL T* gptr(T* p) {
     return (MRLBM_GLOBAL T*)p;
 }
 
+// A uniform global pointer held in scalar registers as an opaque value: an
+// access p[lane offset] is then one instruction with the scalar base and the
+// lane's 32-bit offset, however far p lies from the kernel's other bases
+// (without it, bases that differ by more than the 12-bit immediate offset
+// are formed with 64-bit vector adds in every lane).
+template <typename T>
+__device__ __forceinline__ MRLBM_GLOBAL T* sbase(MRLBM_GLOBAL T* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 // Population ordering of src/includeFiles/velocitySets/D3Q27.inc:44-46;
 // the first 19 entries are the D3Q19 set (D3Q19.inc:17-19).
 __host__ __device__ constexpr int cxv(int i) {

@@ -206,23 +206,26 @@ __device__ __forceinline__ void tile_fetch(const StepArgs& a, const float* gin, 
     tile_coords(a, tile, bx, by, bz, nb);
     in.cls = 0;
     if constexpr (HAS_BC) {
-        if (a.cls) in.cls = gptr(a.cls)[(size_t)tile * T + t];  // uniform: batched lattices may have no class map
+        // uniform test: batched lattices may have no class map
+        if (a.cls) in.cls = (gptr(a.cls) + (size_t)tile * T)[(uint32_t)t];
     }
-    const auto mom = gptr((const typename Codec<S>::T*)a.mom);
-    const size_t mb = (size_t)tile * 10 * T + t;
+    // this tile's moment block: a uniform base (scalar registers) per plane
+    // and the lane's 32-bit offset, so the addresses cost no vector adds
+    const auto mom = gptr((const typename Codec<S>::T*)a.mom) + (size_t)tile * 10 * T;
 #pragma unroll
     // moments are read and written exactly once per step: non-temporal
     // accesses keep them from displacing the ghost-plane lines that
     // neighbouring tiles read (measured +8 % on 256^3 FP32)
     for (int k = 0; k < 10; k++) {
+        const auto plane = mom + (size_t)k * T;
         if constexpr (S == 1) {
             // binary16: the reconstruction reads only the low half of the
             // word, so the high half is left undefined (no zero extension)
             typedef _Float16 __attribute__((ext_vector_type(1))) half1_t;
-            const half1_t x = {__builtin_nontemporal_load((const _Float16*)&mom[mb + (size_t)k * T])};
+            const half1_t x = {__builtin_nontemporal_load((const _Float16*)&plane[(uint32_t)t])};
             in.m[k] = __builtin_bit_cast(uint32_t, __builtin_shufflevector(x, x, 0, -1));
         } else {
-            in.m[k] = (typename Fetched<S>::T)__builtin_nontemporal_load(&mom[mb + (size_t)k * T]);
+            in.m[k] = (typename Fetched<S>::T)__builtin_nontemporal_load(&plane[(uint32_t)t]);
         }
     }
     // wave W owns incoming chunks W, W + NW, ...; every wave fills all
@@ -391,8 +394,7 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* gout, uint
         quantize<S>(m, a, step, node_id, q);
         recon_terms<Q>(m, recon_re0<Q>(m), rt);
     }
-    const size_t mb = (size_t)tile * 10 * T + t;
-    if (!solid) store_node<Q, BX, BY, BZ, S>(a, mb, q);
+    if (!solid) store_tile_node<BX, BY, BZ, S>(a, tile, t, q);
 
     // populations leaving the tile (popSave.inc): every node stages its
     // reconstruction terms in LDS, then whole waves evaluate and write the

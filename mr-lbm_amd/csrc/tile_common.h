@@ -244,6 +244,16 @@ __device__ __forceinline__ void quantize(Node& n, const StepArgs& a, uint32_t st
     }
 }
 
+// Stored moments of node t of tile `tile`: a uniform base per plane and the
+// lane's 32-bit offset (no vector address arithmetic).
+template <int BX, int BY, int BZ, int S>
+__device__ __forceinline__ void store_tile_node(const StepArgs& a, int tile, int t, const typename Codec<S>::T* q) {
+    constexpr size_t T = (size_t)BX * BY * BZ;
+    const auto mom = gptr((typename Codec<S>::T*)a.mom) + (size_t)tile * 10 * T;
+#pragma unroll
+    for (int k = 0; k < 10; k++) __builtin_nontemporal_store(q[k], &sbase(mom + k * T)[(uint32_t)t]);  // streamed once per step
+}
+
 template <int Q, int BX, int BY, int BZ, int S>
 __device__ __forceinline__ void store_node(const StepArgs& a, size_t mb, const typename Codec<S>::T* q) {
     constexpr size_t T = (size_t)BX * BY * BZ;
