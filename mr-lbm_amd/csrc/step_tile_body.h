@@ -209,23 +209,37 @@ __device__ __forceinline__ void tile_fetch(const StepArgs& a, const float* gin, 
         // uniform test: batched lattices may have no class map
         if (a.cls) in.cls = (gptr(a.cls) + (size_t)tile * T)[(uint32_t)t];
     }
-    // this tile's moment block: a uniform base (scalar registers) per plane
-    // and the lane's 32-bit offset, so the addresses cost no vector adds
-    const auto mom = gptr((const typename Codec<S>::T*)a.mom) + (size_t)tile * 10 * T;
+    // this tile's moment block: a uniform base (scalar registers), the
+    // lane's 32-bit byte offset and an immediate. Planes lie PB bytes apart;
+    // those past the 12-bit immediate take a window offset (lane offset +
+    // w * 4 KiB, one 32-bit add, opaque so that the compiler does not fold
+    // it back into 64-bit vector address adds per plane)
+    using MT = typename Codec<S>::T;
+    const auto tb = (const MRLBM_GLOBAL char*)(gptr((const MT*)a.mom) + (size_t)tile * 10 * T);
+    constexpr uint32_t PB = (uint32_t)(T * sizeof(MT));
+    constexpr int PW = PB >= 4096 ? 1 : 4096 / (int)PB;  // planes per window
+    constexpr int NWIN = (10 + PW - 1) / PW;
+    uint32_t woff[NWIN];
+    woff[0] = (uint32_t)t * (uint32_t)sizeof(MT);
+    static_for<NWIN - 1>([&](auto WW) {
+        constexpr int w = decltype(WW)::value + 1;
+        woff[w] = woff[0] + (uint32_t)(w * PW) * PB;
+        asm("" : "+v"(woff[w]));
+    });
 #pragma unroll
     // moments are read and written exactly once per step: non-temporal
     // accesses keep them from displacing the ghost-plane lines that
     // neighbouring tiles read (measured +8 % on 256^3 FP32)
     for (int k = 0; k < 10; k++) {
-        const auto plane = mom + (size_t)k * T;
+        const auto plane = (const MRLBM_GLOBAL MT*)(tb + woff[k / PW] + (k % PW) * PB);
         if constexpr (S == 1) {
             // binary16: the reconstruction reads only the low half of the
             // word, so the high half is left undefined (no zero extension)
             typedef _Float16 __attribute__((ext_vector_type(1))) half1_t;
-            const half1_t x = {__builtin_nontemporal_load((const _Float16*)&plane[(uint32_t)t])};
+            const half1_t x = {__builtin_nontemporal_load((const MRLBM_GLOBAL _Float16*)plane)};
             in.m[k] = __builtin_bit_cast(uint32_t, __builtin_shufflevector(x, x, 0, -1));
         } else {
-            in.m[k] = (typename Fetched<S>::T)__builtin_nontemporal_load(&plane[(uint32_t)t]);
+            in.m[k] = (typename Fetched<S>::T)__builtin_nontemporal_load(plane);
         }
     }
     // wave W owns incoming chunks W, W + NW, ...; every wave fills all
@@ -317,7 +331,10 @@ __device__ __forceinline__ void tile_update(const StepArgs& a, float* gout, uint
         constexpr int i = decltype(I)::value + 1;
         if constexpr (P::x_only(i)) {
             constexpr int ctrl = cxv(i) > 0 ? 0x111 : 0x101;
-            gx[i] = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(g[i]), ctrl, 0xF, 0xF, false));
+            // bound_ctrl: a lane whose source lies outside its DPP row reads
+            // 0 (no old value to initialise); it is a row's first node,
+            // which takes the entering value instead
+            gx[i] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(g[i]), ctrl, 0xF, 0xF, true));
         }
     });
     // each phase: write its populations, barrier, pull them (D3Q27 runs two
